@@ -1,0 +1,162 @@
+/*
+ * stab_hip.h -- C ABI of the MI355X (gfx950) implementation of stab's
+ * Chebyshev-collocation stability solve (reference: sscollis/stab, Fortran).
+ *
+ * The reference's operator interface for this path is the pair of Fortran
+ * subroutines temporal(name, ind) (temporal.f90:2) and spatial(name, ind)
+ * (spatial.f90:2).  They take every input from module `stuff`
+ * (stuff.f90:9-55) plus the mean-flow file profile.<ind> (getmean.f90:27-32),
+ * build a dense complex pencil and call LAPACK on one core
+ * (ZGESV+ZGEEV at temporal.f90:774,803; ZGETRF/ZGETRS+ZGEEV at
+ * spatial.f90:978,996,1002,1043).  This library replaces that arithmetic:
+ * the pencil is assembled on the GPU and the eigenpairs nearest a shift are
+ * found by a blocked LU of the shifted pencil plus shift-invert Arnoldi.
+ *
+ * Conventions: plain C types only; complex numbers are interleaved
+ * (re, im) doubles, Fortran column-major arrays, 0-based indices.  The caller
+ * owns every host buffer; a context owns device memory and one HIP stream.
+ * Every entry point returns a status (STAB_OK or a negative STAB_E*); per-point
+ * results carry LAPACK-style info codes.  Nothing calls exit/stop.
+ * One context is used by one host thread at a time.
+ */
+#ifndef STAB_HIP_H
+#define STAB_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct { double re, im; } stab_z;
+
+/* Status codes returned by every entry point. */
+#define STAB_OK            0
+#define STAB_E_ARG        -1   /* bad argument (stab_last_error names it) */
+#define STAB_E_HIP        -2   /* HIP runtime error */
+#define STAB_E_NOMEM      -3   /* device allocation failed */
+#define STAB_E_NOPROFILE  -4   /* point refers to a profile slot never set */
+
+/* Per-point info codes (stab_solve info[]) */
+#define STAB_INFO_OK         0
+#define STAB_INFO_SINGULAR   1   /* exact zero pivot in the LU of the shifted pencil */
+#define STAB_INFO_NOCONV     2   /* not every wanted pair met the residual tolerance */
+
+/* Solve modes */
+#define STAB_MODE_PLAIN_F64  0   /* FP64 LU + FP64 shift-invert Arnoldi */
+#define STAB_MODE_MIXED      1   /* FP32 LU + FP64-residual refined eigenpairs */
+
+/* Matrices returned by stab_pencil (reference form, for parity checks) */
+#define STAB_MAT_A0  0   /* temporal A0  (temporal.f90:664-752) */
+#define STAB_MAT_B0  1   /* temporal B0  (temporal.f90:628-662) */
+#define STAB_MAT_C0  2   /* spatial  C0  (spatial.f90:700-835)  */
+#define STAB_MAT_C1  3   /* spatial  C1  (spatial.f90:837-938)  */
+#define STAB_MAT_C2  4   /* spatial  C2  (spatial.f90:940-959)  */
+#define STAB_MAT_SHIFTED 5 /* the matrix factored by stab_solve: A0 - sigma B0
+                              (temporal) or C0 + sigma C1 + sigma^2 C2 (spatial),
+                              with Dirichlet rows replaced by unit rows */
+
+/*
+ * One parameter point: the subset of module `stuff` (stuff.f90:9-55) that
+ * temporal/spatial read, plus the shift.  Field meanings follow input.f90:
+ *   mattyp  0 constant mu, 1 Sutherland (input.f90:14-32)
+ *   Te      edge temperature as input.f90 computes it: T0 as entered when
+ *           mattyp = 1 (input.f90:24-25 runs before Ma is read), 0 otherwise
+ *   ny, yi, ymax   grid (sgengrid.f90:26-45); yi must be nonzero
+ *   top, wallt     boundary-condition flags (input.f90:57-59)
+ *   alpha, beta    temporal wavenumbers; omega, beta for spatial
+ *   shift          sigma: a guess of omega (temporal) or alpha (spatial),
+ *                  i.e. the value the reference's getevec -v reads
+ *                  (getevec.f90:160-173, thesis/TStest/run.sh:13-17)
+ *   profile        slot set by stab_profile_set (profile.<ind>)
+ *   nev            eigenpairs wanted nearest the shift (<= STAB_MAX_NEV)
+ */
+#define STAB_MAX_NEV 16
+typedef struct {
+  int32_t itype;     /* 1 temporal, 2 spatial */
+  int32_t ny;
+  int32_t mattyp;
+  int32_t wallt;     /* 0 or 2 */
+  int32_t top;       /* 0 or 1 (spatial) */
+  int32_t profile;
+  int32_t nev;
+  int32_t ider;      /* must be 1: derivatives by collocation (temporal.f90:99-103) */
+  double Ma, Re, Pr, Te;
+  double yi, ymax;
+  stab_z alpha, beta, omega;
+  stab_z shift;
+} stab_point;
+
+typedef struct stab_ctx stab_ctx;
+
+/* Solver statistics of the last stab_solve call (device-timed, ms). */
+typedef struct {
+  double ms_prep, ms_assemble, ms_lu, ms_eig, ms_total;
+  int64_t lu_flops;       /* algorithmic (8/3) n^3 per point, summed */
+  int64_t assemble_bytes; /* algorithmic bytes written/read by assembly */
+  int32_t npts, nconv;    /* points, points whose nev pairs all converged */
+  int32_t restarts_max;
+} stab_stats;
+
+/* Context on HIP device `device`.  New in this library (the reference keeps
+ * its state in module globals, stuff.f90:2-61). */
+int stab_ctx_create(stab_ctx **ctx, int device);
+int stab_ctx_destroy(stab_ctx *ctx);
+const char *stab_last_error(const stab_ctx *ctx);
+
+/* Mean-flow profile slot `slot`: nym rows of y and vt (nym x 5, column-major:
+ * rho, u, v, w, T).  v is forced to zero and the cubic-spline second
+ * derivatives are formed on the device, as getmean.f90:56-86 does after
+ * reading profile.<ind>. */
+int stab_profile_set(stab_ctx *ctx, int slot, int nym, const double *y, const double *vt);
+
+/* Grid, interpolated mean flow and Chebyshev operators of one point, as the
+ * device builds them (sgengrid.f90:15-45, getmean.f90:95-112, chebyd.f90:2-60,
+ * temporal.f90:136).  y/eta/deta/d2eta/vm may be NULL; vm is ny x 5
+ * column-major; d1/d2 are ny x ny column-major or NULL. */
+int stab_grid(stab_ctx *ctx, const stab_point *pt, double *y, double *eta, double *deta,
+              double *d2eta, double *vm, double *d1, double *d2);
+
+/* Assemble matrix `which` (STAB_MAT_*) of one point on the device and copy
+ * it to `out` (n x n column-major, n = 5 ny).  Replaces the fill loops of
+ * temporal.f90:600-752 / spatial.f90:685-959. */
+int stab_pencil(stab_ctx *ctx, const stab_point *pt, int which, stab_z *out);
+
+/* Batched solve: for each point, the nev eigenvalues nearest pt.shift and
+ * their eigenvectors.  Replaces ZGESV+ZGEEV (temporal.f90:771-810) and
+ * ZGETRF/ZGETRS+ZGEEV (spatial.f90:972-1058) for the wanted part of the
+ * spectrum, with the eigenvector scaling of temporal.f90:867-879 /
+ * getevec.f90:179-189 applied to every returned vector.
+ *   eval   npts*STAB_MAX_NEV values (point p, pair k at p*STAB_MAX_NEV+k),
+ *          sorted by distance to the shift
+ *   evec   NULL, or for point p a block of nvec_p*STAB_MAX_NEV values at offset
+ *          sum_{q<p} nvec_q*STAB_MAX_NEV, nvec = 5 ny (temporal) or 10 ny
+ *          (spatial: z = [phi; alpha phi], as the companion of
+ *          spatial.f90:1007-1016 returns), column k = pair k
+ *   resid  npts*STAB_MAX_NEV relative residuals
+ *          ||(A - w B) x|| / ((||A|| + |w| ||B||) ||x||) in FP64 (Frobenius-type
+ *          norms of the pencil's coefficient operators), or NULL
+ *   info   npts per-point STAB_INFO_* codes
+ */
+int stab_solve(stab_ctx *ctx, int npts, const stab_point *pts, int mode, stab_z *eval,
+               stab_z *evec, double *resid, int32_t *info, stab_stats *stats);
+
+/* Factor the shifted matrix of one point (STAB_MAT_SHIFTED) and solve
+ * M x = b (n = 5 ny entries).  Replaces the ZGETRF/ZGETRS pair of
+ * spatial.f90:978,996 (one right-hand side).  mode STAB_MODE_MIXED factors the
+ * row/column-equilibrated matrix in FP32 and applies ir_steps FP64
+ * iterative-refinement steps (ir_steps < 0: adaptive, at most -ir_steps, until
+ * the correction is below 1e-14 relative); STAB_MODE_PLAIN_F64 factors in FP64 (ir_steps
+ * then refine against the same FP64 matrix).  info: STAB_INFO_SINGULAR on an
+ * exact zero pivot. */
+int stab_shifted_solve(stab_ctx *ctx, const stab_point *pt, int mode, int ir_steps, const stab_z *b, stab_z *x,
+                       int32_t *info);
+
+/* Size in bytes of the evec buffer stab_solve needs for these points. */
+size_t stab_evec_bytes(int npts, const stab_point *pts);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* STAB_HIP_H */

@@ -1,0 +1,103 @@
+// Device-side types shared by the kernels of libstab_hip (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define STAB_NDOF 5
+
+template <typename T> struct cx { T x, y; };
+typedef cx<double> zd;
+typedef cx<float> zf;
+
+template <typename T> __host__ __device__ inline cx<T> mk(T a, T b) { cx<T> r; r.x = a; r.y = b; return r; }
+template <typename T> __host__ __device__ inline cx<T> operator+(cx<T> a, cx<T> b) { return mk(a.x + b.x, a.y + b.y); }
+template <typename T> __host__ __device__ inline cx<T> operator-(cx<T> a, cx<T> b) { return mk(a.x - b.x, a.y - b.y); }
+template <typename T> __host__ __device__ inline cx<T> operator-(cx<T> a) { return mk(-a.x, -a.y); }
+template <typename T> __host__ __device__ inline cx<T> operator*(cx<T> a, cx<T> b) {
+  return mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// complex * real, written as the reference's Fortran promotion does it:
+// (a.x + i a.y)(s + 0i) = (a.x s - a.y 0) + i(a.x 0 + a.y s), which is exactly (a.x s, a.y s).
+template <typename T> __host__ __device__ inline cx<T> operator*(cx<T> a, T s) { return mk(a.x * s, a.y * s); }
+template <typename T> __host__ __device__ inline cx<T> operator*(T s, cx<T> a) { return mk(a.x * s, a.y * s); }
+template <typename T> __host__ __device__ inline cx<T> conj(cx<T> a) { return mk(a.x, -a.y); }
+template <typename T> __host__ __device__ inline T abs2(cx<T> a) { return a.x * a.x + a.y * a.y; }
+template <typename T> __host__ __device__ inline T cabs(cx<T> a) { return hypot(a.x, a.y); }
+template <typename T> __host__ __device__ inline T cabs1(cx<T> a) { return fabs(a.x) + fabs(a.y); }
+// Smith's algorithm for complex division
+template <typename T> __host__ __device__ inline cx<T> cdiv(cx<T> a, cx<T> b) {
+  if (fabs(b.x) >= fabs(b.y)) {
+    T r = b.y / b.x, d = b.x + b.y * r;
+    return mk((a.x + a.y * r) / d, (a.y - a.x * r) / d);
+  } else {
+    T r = b.x / b.y, d = b.y + b.x * r;
+    return mk((a.x * r + a.y) / d, (a.y * r - a.x) / d);
+  }
+}
+// a - b*c and a + b*c in complex FMA form
+template <typename T> __host__ __device__ inline cx<T> cfms(cx<T> a, cx<T> b, cx<T> c) {
+  a.x = fma(-b.x, c.x, a.x); a.x = fma(b.y, c.y, a.x);
+  a.y = fma(-b.x, c.y, a.y); a.y = fma(-b.y, c.x, a.y);
+  return a;
+}
+template <typename T> __host__ __device__ inline cx<T> cfma(cx<T> a, cx<T> b, cx<T> c) {
+  a.x = fma(b.x, c.x, a.x); a.x = fma(-b.y, c.y, a.x);
+  a.y = fma(b.x, c.y, a.y); a.y = fma(b.y, c.x, a.y);
+  return a;
+}
+// a + conj(b)*c
+template <typename T> __host__ __device__ inline cx<T> cfmac(cx<T> a, cx<T> b, cx<T> c) {
+  a.x = fma(b.x, c.x, a.x); a.x = fma(b.y, c.y, a.x);
+  a.y = fma(b.x, c.y, a.y); a.y = fma(-b.y, c.x, a.y);
+  return a;
+}
+template <typename T> __host__ __device__ inline cx<T> czero() { return mk<T>(0, 0); }
+
+template <typename To, typename Ti> __host__ __device__ inline cx<To> cvt(cx<Ti> a) { return mk<To>((To)a.x, (To)a.y); }
+
+// Per-node coefficient slots (each a 5x5 complex block, row-major p*5+q):
+//   temporal: BH  = (B - i a Vxy - i b Vyz) eta' - Vyy eta''   (temporal.f90:609-617)
+//             DH  = D + i a A + i b C + a^2 Vxx + a b Vxz + b^2 Vzz  (temporal.f90:604-605)
+//             VYY = Vyy eta'^2 (real)                          (temporal.f90:616)
+//             G   = G (real)                                    (temporal.f90:313-531)
+//   spatial:  BH  = (B - i b Vyz) eta' - Vyy eta''              (spatial.f90:689-697)
+//             DH  = D + i b C + b^2 Vzz - i w G                 (spatial.f90:687, 808-809)
+//             VYY, G as above
+//             BH1 = -i Vxy eta'                                 (spatial.f90:841-849)
+//             DH1 = i A + b Vxz                                 (spatial.f90:839)
+//             VXX = Vxx (real)                                  (spatial.f90:942-949)
+enum { SLOT_BH = 0, SLOT_DH = 1, SLOT_VYY = 2, SLOT_G = 3, SLOT_BH1 = 4, SLOT_DH1 = 5, SLOT_VXX = 6, NSLOT = 7 };
+#define NODE_STRIDE (NSLOT * 25)
+
+// Device-side description of one point of a batch.
+struct PointDev {
+  int itype, ny, n, npad;        // n = 5 ny; npad = n rounded up to the LU block
+  int mattyp, wallt, top, navier;
+  int nev, nvec, prof, nym;      // nvec = n (temporal) or 2n (spatial)
+  int op_off;                    // offset (doubles) of D1/D2/dt2w for this ny
+  int pad0;
+  double Ma, Re, Pr, Te, yi, ymax;
+  double dm1, dm2, dm3;          // datmat (input.f90:28-35)
+  double rmue, rlme, cone;       // edge properties (input.f90:43-44)
+  zd alpha, beta, omega, shift;
+  long long prof_off;            // offset (doubles) of the profile rows
+  long long coef_off;            // offset (zd) of the node coefficients
+  long long mat_off;             // offset (elements) of the LU matrix
+  long long vec_off;             // offset (zd) of the Krylov basis
+  long long small_off;           // offset (zd) of H / Q scratch
+  long long piv_off;             // offset (int) of pivots
+  long long blk_off;             // offset (elements) of the Linv/Uinv blocks
+  long long scale_off;           // offset (double) of equilibration scales (2 npad)
+  long long out_off;             // offset (zd) of the output eigenvectors
+};
+
+// Matrix forms produced by the assembly kernel
+enum {
+  FORM_A0 = 0, FORM_B0 = 1, FORM_C0 = 2, FORM_C1 = 3, FORM_C2 = 4, FORM_SHIFTED = 5,
+  // temporal pencil as the solver uses it: Dirichlet rows of A become unit rows
+  // and those of B zero rows, which maps the reference's spurious omega = 0
+  // eigenvalues of those rows to infinity; every other eigenpair is unchanged.
+  FORM_AM = 6, FORM_BM = 7,
+};
+
+__device__ inline int wave_lane() { return threadIdx.x & 63; }

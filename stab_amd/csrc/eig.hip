@@ -1,0 +1,722 @@
+// Shift-invert Krylov-Schur eigensolver kernels (one work-group per point).
+//
+// Operator (temporal): T v = (A - sigma B)^-1 B v on the solver pencil, whose
+// eigenvalues theta = 1/(omega - sigma) are largest for the omega nearest the
+// shift.  Operator (spatial): the linearisation
+//     [[C0,0],[0,I]] z = alpha [[-C1,-C2],[I,0]] z,  z = [phi; alpha phi]
+// of the reference's companion (spatial.f90:982-1016); its shift-invert
+// product reduces to one n x n solve with L(sigma) = C0 + sigma C1 + sigma^2 C2:
+//     u = L(sigma)^-1 (-C1 z1 - C2 (z2 + sigma z1)),  T z = [u; z1 + sigma u].
+// Solves use the LU factors (FP64, or equilibrated FP32 followed by FP64
+// iterative-refinement steps whose residuals use the matrix-free pencil).
+#include "pencil.h"
+
+#define KS_MMAX 64
+#define KS_NEVMAX 16
+
+// Per-point Krylov-Schur state in global memory (small_off points here):
+//   H  (KS_MMAX+1) x KS_MMAX, column-major, ld = KS_MMAX+1
+//   Q  KS_MMAX x KS_MMAX (ld = KS_MMAX), Y KS_MMAX x KS_NEVMAX, theta[KS_MMAX]
+#define SM_H 0
+#define SM_Q ((KS_MMAX + 1) * KS_MMAX)
+#define SM_Y (SM_Q + KS_MMAX * KS_MMAX)
+#define SM_TH (SM_Y + KS_MMAX * KS_NEVMAX)
+#define SM_SIZE (SM_TH + KS_MMAX)
+
+struct KSState {
+  int j0;        // first column to expand
+  int k;         // kept columns after the last restart
+  int done;      // 1 when every wanted pair has converged
+  int cycles;
+  double resmax; // max relative Ritz residual estimate of the wanted pairs
+};
+
+// ---------------------------------------------------------------------------
+// y <- M^-1 y with the blocked factors of k_lu (scales applied by the caller).
+// y, tmp in LDS (tmp: NB entries).  All threads call.
+// ---------------------------------------------------------------------------
+template <typename T, int NB, int NT>
+__device__ void lu_solve(const PointDev &P, const cx<T> *__restrict__ A, const int *__restrict__ ipiv,
+                         const cx<T> *__restrict__ Lb, const cx<T> *__restrict__ Ub, zd *y, zd *tmp) {
+  const int n = P.npad, lda = P.npad, t = threadIdx.x;
+  for (int k0 = 0; k0 < n; k0 += NB) {
+    if (t == 0) {
+      for (int c = 0; c < NB; ++c) {
+        const int p = ipiv[k0 + c];
+        if (p != k0 + c && p > k0 + c && p < n) { const zd a = y[k0 + c]; y[k0 + c] = y[p]; y[p] = a; }
+      }
+    }
+    __syncthreads();
+    if (t < NB) {
+      zd s = czero<double>();
+      for (int k = 0; k <= t; ++k) s = cfma(s, cvt<double>(Lb[(size_t)k0 * NB + t * NB + k]), y[k0 + k]);
+      tmp[t] = s;
+    }
+    __syncthreads();
+    if (t < NB) y[k0 + t] = tmp[t];
+    for (int r = k0 + NB + t; r < n; r += NT) {
+      zd acc = y[r];
+#pragma unroll 8
+      for (int c = 0; c < NB; ++c) acc = cfms(acc, cvt<double>(A[(size_t)(k0 + c) * lda + r]), tmp[c]);
+      y[r] = acc;
+    }
+    __syncthreads();
+  }
+  for (int k0 = n - NB; k0 >= 0; k0 -= NB) {
+    if (t < NB) {
+      zd s = czero<double>();
+      for (int k = t; k < NB; ++k) s = cfma(s, cvt<double>(Ub[(size_t)k0 * NB + t * NB + k]), y[k0 + k]);
+      tmp[t] = s;
+    }
+    __syncthreads();
+    if (t < NB) y[k0 + t] = tmp[t];
+    for (int r = t; r < k0; r += NT) {
+      zd acc = y[r];
+#pragma unroll 8
+      for (int c = 0; c < NB; ++c) acc = cfms(acc, cvt<double>(A[(size_t)(k0 + c) * lda + r]), tmp[c]);
+      y[r] = acc;
+    }
+    __syncthreads();
+  }
+}
+
+// Block-wide reduction of NV complex partial sums; result broadcast to all.
+template <int NT, int NV>
+__device__ void block_reduce(zd (&v)[NV], zd *red) {
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    for (int off = 32; off > 0; off >>= 1) {
+      v[k].x += __shfl_xor(v[k].x, off);
+      v[k].y += __shfl_xor(v[k].y, off);
+    }
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) red[w * NV + k] = v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    zd s = czero<double>();
+    for (int ww = 0; ww < NT / 64; ++ww) s = s + red[ww * NV + k];
+    v[k] = s;
+  }
+  __syncthreads();
+}
+
+template <int NT>
+__device__ double block_norm2(const zd *x, int len, zd *red) {
+  zd v[1] = {czero<double>()};
+  for (int r = threadIdx.x; r < len; r += NT) v[0].x += abs2(x[r]);
+  block_reduce<NT, 1>(v, red);
+  return v[0].x;
+}
+
+// Shared-memory layout of the operator kernels (dynamic LDS, zd units).
+struct OpLds {
+  zd *y, *rhs, *res, *X1, *X2, *z, *red, *tmp;
+  __device__ OpLds(zd *base, const PointDev &P, int npadmax, int nymax) {
+    y = base;
+    rhs = y + npadmax;
+    res = rhs + npadmax;
+    X1 = res + npadmax;
+    X2 = X1 + 5 * nymax;
+    z = X2 + 5 * nymax;
+    red = z + 2 * npadmax;
+    tmp = red + 64;
+  }
+};
+
+__host__ __device__ inline size_t op_lds_bytes(int npadmax, int nymax) {
+  return sizeof(zd) * ((size_t)3 * npadmax + 10 * nymax + 2 * npadmax + 64 + 64);
+}
+
+// y = M^-1 rhs in FP64 quality: scaled solve with the factors, then
+// iterative-refinement steps with the FP64 matrix-free shifted pencil.
+// ir > 0: exactly ir steps.  ir < 0: adaptive, at most -ir steps, stopping once
+// the correction is below 1e-14 of the solution or stops shrinking by half
+// (the FP64 rounding floor of the residual has been reached).  The contraction
+// per step is about kappa(M) u32.
+template <typename T, int NB, int NT>
+__device__ void shifted_solve(const PointDev &P, const cx<T> *A, const int *ipiv, const cx<T> *Lb, const cx<T> *Ub,
+                              const double *dr, const double *dc, const zd *coef, const double *ops, int ir,
+                              OpLds &L) {
+  const int n = P.npad, t = threadIdx.x;
+  for (int r = t; r < n; r += NT) L.y[r] = (r < P.n) ? L.rhs[r] * dr[r] : czero<double>();
+  __syncthreads();
+  lu_solve<T, NB, NT>(P, A, ipiv, Lb, Ub, L.y, L.tmp);
+  for (int r = t; r < n; r += NT) L.y[r] = L.y[r] * dc[r];
+  __syncthreads();
+  const bool adaptive = ir < 0;
+  const int steps = adaptive ? -ir : ir;
+  double dprev = 0.0;
+  for (int s = 0; s < steps; ++s) {
+    pencil_apply<NT>(P, coef, ops, FORM_SHIFTED, P.shift, L.y, L.res, L.X1, L.X2, L.red);
+    for (int r = t; r < n; r += NT) L.res[r] = (r < P.n) ? (L.rhs[r] - L.res[r]) * dr[r] : czero<double>();
+    __syncthreads();
+    lu_solve<T, NB, NT>(P, A, ipiv, Lb, Ub, L.res, L.tmp);
+    zd nrm[2] = {czero<double>(), czero<double>()};
+    for (int r = t; r < n; r += NT) {
+      const zd d = L.res[r] * dc[r];
+      L.y[r] = L.y[r] + d;
+      nrm[0].x += abs2(d);
+      nrm[1].x += abs2(L.y[r]);
+    }
+    if (adaptive) {
+      block_reduce<NT, 2>(nrm, L.red);
+      if (nrm[0].x <= 1e-28 * nrm[1].x || (s > 0 && nrm[0].x > 0.25 * dprev)) break;
+      dprev = nrm[0].x;
+    } else {
+      __syncthreads();
+    }
+  }
+}
+
+// w = T v for point P; v read from global (length nvec), w left in L.z.
+template <typename T, int NB, int NT>
+__device__ void apply_op(const PointDev &P, const cx<T> *A, const int *ipiv, const cx<T> *Lb, const cx<T> *Ub,
+                         const double *dr, const double *dc, const zd *coef, const double *ops, int ir,
+                         const zd *v, OpLds &L) {
+  const int n = P.n, t = threadIdx.x;
+  if (P.itype == 1) {
+    for (int r = t; r < n; r += NT) L.z[r] = v[r];
+    __syncthreads();
+    pencil_apply<NT>(P, coef, ops, FORM_BM, P.shift, L.z, L.rhs, L.X1, L.X2, L.red);
+    shifted_solve<T, NB, NT>(P, A, ipiv, Lb, Ub, dr, dc, coef, ops, ir, L);
+    for (int r = t; r < n; r += NT) L.z[r] = L.y[r];
+    __syncthreads();
+  } else {
+    const zd sg = P.shift;
+    // z holds [z1; z2 + sigma z1]
+    for (int r = t; r < n; r += NT) { L.z[r] = v[r]; L.z[n + r] = v[n + r] + sg * v[r]; }
+    __syncthreads();
+    pencil_apply<NT>(P, coef, ops, FORM_C1, sg, L.z, L.rhs, L.X1, L.X2, L.red);
+    pencil_apply<NT>(P, coef, ops, FORM_C2, sg, L.z + n, L.res, L.X1, L.X2, L.red);
+    for (int r = t; r < n; r += NT) L.rhs[r] = czero<double>() - L.rhs[r] - L.res[r];
+    __syncthreads();
+    shifted_solve<T, NB, NT>(P, A, ipiv, Lb, Ub, dr, dc, coef, ops, ir, L);
+    for (int r = t; r < n; r += NT) { L.z[n + r] = L.z[r] + sg * L.y[r]; L.z[r] = L.y[r]; }
+    __syncthreads();
+  }
+}
+
+// Deterministic start vector: T applied to a hashed pseudo-random vector.
+__device__ inline double hash01(unsigned a, unsigned b) {
+  unsigned h = a * 0x9E3779B1u ^ (b + 0x7F4A7C15u) * 0x85EBCA77u;
+  h ^= h >> 15; h *= 0x2C1B3C6Du; h ^= h >> 12; h *= 0x297A2D39u; h ^= h >> 15;
+  return (double)h * (1.0 / 4294967296.0) - 0.5;
+}
+
+// ---------------------------------------------------------------------------
+// Arnoldi expansion from column j0 to m (CGS2).  V column-major, ld = ldv.
+// ---------------------------------------------------------------------------
+template <typename T, int NB, int NT>
+__global__ __launch_bounds__(NT) void k_arnoldi(const PointDev *__restrict__ pts, const cx<T> *__restrict__ mats,
+                                                const int *__restrict__ pivs, const cx<T> *__restrict__ blks,
+                                                const double *__restrict__ scales, const zd *__restrict__ coef,
+                                                const double *__restrict__ ops, zd *__restrict__ Vall,
+                                                zd *__restrict__ small, KSState *__restrict__ st, int m, int ir,
+                                                int npadmax, int nymax, int start) {
+  extern __shared__ zd lds[];
+  const PointDev &P = pts[blockIdx.x];
+  KSState S = st[blockIdx.x];
+  if (S.done) return;
+  OpLds L(lds, P, npadmax, nymax);
+  const int t = threadIdx.x, nv = P.nvec;
+  const size_t ldv = (size_t)nv;
+  zd *V = Vall + P.vec_off;
+  zd *H = small + P.small_off + SM_H;
+  const int ldh = KS_MMAX + 1;
+  const cx<T> *A = mats + P.mat_off;
+  const int *ipiv = pivs + P.piv_off;
+  const cx<T> *Lb = blks + P.blk_off, *Ub = Lb + (size_t)P.npad * NB;
+  const double *dr = scales + P.scale_off, *dc = dr + P.npad;
+
+  if (start) {
+    // v0 = T r / ||T r||, r pseudo-random
+    for (int r = t; r < nv; r += NT) V[r] = mk(hash01(blockIdx.x, 2 * r), hash01(blockIdx.x, 2 * r + 1));
+    __threadfence_block();
+    __syncthreads();
+    apply_op<T, NB, NT>(P, A, ipiv, Lb, Ub, dr, dc, coef, ops, ir, V, L);
+    const double nrm = sqrt(block_norm2<NT>(L.z, nv, L.red));
+    for (int r = t; r < nv; r += NT) V[r] = L.z[r] * (1.0 / nrm);
+    for (int e = t; e < ldh * KS_MMAX; e += NT) H[e] = czero<double>();
+    __syncthreads();
+    S.j0 = 0; S.k = 0;
+  }
+  for (int j = S.j0; j < m; ++j) {
+    apply_op<T, NB, NT>(P, A, ipiv, Lb, Ub, dr, dc, coef, ops, ir, V + (size_t)j * ldv, L);
+    // two passes of classical Gram-Schmidt against V[:, 0..j]
+    for (int pass = 0; pass < 2; ++pass) {
+      for (int l0 = 0; l0 <= j; l0 += 8) {
+        zd h[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) h[k] = czero<double>();
+        for (int r = t; r < nv; r += NT) {
+          const zd w = L.z[r];
+#pragma unroll
+          for (int k = 0; k < 8; ++k)
+            if (l0 + k <= j) h[k] = cfmac(h[k], V[(size_t)(l0 + k) * ldv + r], w);
+        }
+        block_reduce<NT, 8>(h, L.red);
+        for (int r = t; r < nv; r += NT) {
+          zd w = L.z[r];
+#pragma unroll
+          for (int k = 0; k < 8; ++k)
+            if (l0 + k <= j) w = cfms(w, V[(size_t)(l0 + k) * ldv + r], h[k]);
+          L.z[r] = w;
+        }
+        if (t < 8 && l0 + t <= j) {
+          zd hv = h[0];
+#pragma unroll
+          for (int k = 0; k < 8; ++k) if (k == t) hv = h[k];
+          H[(size_t)j * ldh + l0 + t] = H[(size_t)j * ldh + l0 + t] + hv;
+        }
+        __syncthreads();
+      }
+    }
+    double nrm = sqrt(block_norm2<NT>(L.z, nv, L.red));
+    if (!(nrm > 1e-300)) nrm = 1e-300;
+    for (int r = t; r < nv; r += NT) V[(size_t)(j + 1) * ldv + r] = L.z[r] * (1.0 / nrm);
+    if (t == 0) H[(size_t)j * ldh + j + 1] = mk(nrm, 0.0);
+    __syncthreads();
+  }
+  if (t == 0) { S.j0 = m; st[blockIdx.x] = S; }
+}
+
+// ---------------------------------------------------------------------------
+// Small dense eigenproblem of H_m (one wave): Householder reduction to
+// Hessenberg form, complex single-shift QR to Schur form T = Q^H H Q, then the
+// `keep` eigenvalues of largest modulus are moved to the leading positions in
+// decreasing modulus.  Ritz vectors of the first nev, residual estimates and
+// convergence are written back.
+// ---------------------------------------------------------------------------
+__device__ inline void givens(zd x, zd y, double &c, zd &s) {
+  const double ax = cabs(x), ay = cabs(y);
+  if (ay == 0.0) { c = 1.0; s = czero<double>(); return; }
+  if (ax == 0.0) { c = 0.0; s = conj(y) * (1.0 / ay); return; }
+  const double r = hypot(ax, ay);
+  c = ax / r;
+  s = (x * (1.0 / ax)) * conj(y) * (1.0 / r);
+}
+
+__global__ __launch_bounds__(64) void k_schur(const PointDev *__restrict__ pts, zd *__restrict__ small,
+                                              KSState *__restrict__ st, int m, int keep, double tol) {
+  extern __shared__ zd sl[];
+  const PointDev &P = pts[blockIdx.x];
+  KSState S = st[blockIdx.x];
+  if (S.done) return;
+  zd *Hs = sl, *Q = Hs + m * m, *v = Q + m * m, *zz = v + m;   // zz: KS_NEVMAX * m
+  zd *Hg = small + P.small_off + SM_H;
+  zd *Qg = small + P.small_off + SM_Q;
+  zd *Yg = small + P.small_off + SM_Y;
+  zd *THg = small + P.small_off + SM_TH;
+  const int ldh = KS_MMAX + 1, t = threadIdx.x;
+  const int nev = P.nev;
+  for (int e = t; e < m * m; e += 64) {
+    const int i = e % m, j = e / m;
+    Hs[i * m + j] = Hg[(size_t)j * ldh + i];   // row-major in LDS
+    Q[i * m + j] = (i == j) ? mk(1.0, 0.0) : czero<double>();
+  }
+  const zd hlast = Hg[(size_t)(m - 1) * ldh + m];
+  __syncthreads();
+  // --- Householder reduction to Hessenberg form
+  for (int k = 0; k < m - 2; ++k) {
+    const int len = m - k - 1;
+    double nx = 0.0;
+    for (int i = 0; i < len; ++i) nx += abs2(Hs[(k + 1 + i) * m + k]);
+    double below = nx - abs2(Hs[(k + 1) * m + k]);
+    if (below <= 1e-300 * (nx + 1e-300)) continue;     // already reduced
+    nx = sqrt(nx);
+    const zd x0 = Hs[(k + 1) * m + k];
+    const double ax0 = cabs(x0);
+    const zd ph = (ax0 > 0.0) ? x0 * (1.0 / ax0) : mk(1.0, 0.0);
+    const zd alpha = ph * (-nx);
+    // v = x - alpha e1, normalised
+    for (int i = t; i < len; i += 64) v[i] = (i == 0) ? x0 - alpha : Hs[(k + 1 + i) * m + k];
+    __syncthreads();
+    double vn = 0.0;
+    for (int i = 0; i < len; ++i) vn += abs2(v[i]);
+    vn = sqrt(vn);
+    __syncthreads();
+    for (int i = t; i < len; i += 64) v[i] = v[i] * (1.0 / vn);
+    __syncthreads();
+    // H <- (I - 2 v v^H) H  on rows k+1.., columns k..m-1
+    for (int c = k + t; c < m; c += 64) {
+      zd s = czero<double>();
+      for (int i = 0; i < len; ++i) s = cfmac(s, v[i], Hs[(k + 1 + i) * m + c]);
+      for (int i = 0; i < len; ++i) Hs[(k + 1 + i) * m + c] = cfms(Hs[(k + 1 + i) * m + c], v[i], s * 2.0);
+    }
+    __syncthreads();
+    // H <- H (I - 2 v v^H) on columns k+1.., all rows; Q likewise
+    for (int rr = t; rr < 2 * m; rr += 64) {
+      zd *row = (rr < m) ? &Hs[rr * m] : &Q[(rr - m) * m];
+      zd s = czero<double>();
+      for (int i = 0; i < len; ++i) s = cfma(s, row[k + 1 + i], v[i]);
+      for (int i = 0; i < len; ++i) row[k + 1 + i] = cfms(row[k + 1 + i], s * 2.0, conj(v[i]));
+    }
+    __syncthreads();
+    if (t == 0)
+      for (int i = 2; i < len + 1; ++i) Hs[(k + i) * m + k] = czero<double>();
+    __syncthreads();
+  }
+  // --- complex QR iterations to Schur form (full T and Q)
+  const double eps = 2.220446049250313e-16;
+  int hi = m - 1, iter = 0, total = 0;
+  while (hi > 0 && total < 60 * m) {
+    int lo = hi;
+    while (lo > 0) {
+      const double sub = cabs1(Hs[lo * m + lo - 1]);
+      const double dia = cabs1(Hs[lo * m + lo]) + cabs1(Hs[(lo - 1) * m + lo - 1]);
+      if (sub <= eps * dia || sub < 1e-300) break;
+      --lo;
+    }
+    if (lo > 0) {
+      __syncthreads();
+      if (t == 0) Hs[lo * m + lo - 1] = czero<double>();
+      __syncthreads();
+    }
+    if (lo == hi) { --hi; iter = 0; continue; }
+    ++iter; ++total;
+    zd mu;
+    const zd a = Hs[(hi - 1) * m + hi - 1], b = Hs[(hi - 1) * m + hi], c = Hs[hi * m + hi - 1], d = Hs[hi * m + hi];
+    if (iter % 11 == 10) {
+      mu = d + mk(0.75 * cabs(c), 0.0);
+    } else {
+      const zd half = (a - d) * 0.5;
+      zd disc = half * half + b * c;
+      // complex sqrt
+      const double r = cabs(disc);
+      double sr = sqrt(0.5 * (r + fabs(disc.x)));
+      zd sq;
+      if (sr == 0.0) sq = czero<double>();
+      else if (disc.x >= 0) sq = mk(sr, disc.y / (2.0 * sr));
+      else sq = mk(fabs(disc.y) / (2.0 * sr), disc.y >= 0 ? sr : -sr);
+      const zd m1 = (a + d) * 0.5 + sq, m2 = (a + d) * 0.5 - sq;
+      mu = (cabs(m1 - d) < cabs(m2 - d)) ? m1 : m2;
+    }
+    for (int k = lo; k < hi; ++k) {
+      zd x, y;
+      if (k == lo) { x = Hs[lo * m + lo] - mu; y = Hs[(lo + 1) * m + lo]; }
+      else { x = Hs[k * m + k - 1]; y = Hs[(k + 1) * m + k - 1]; }
+      double cg; zd sg;
+      givens(x, y, cg, sg);
+      __syncthreads();
+      const int c0 = (k > lo) ? k - 1 : lo;
+      for (int jj = c0 + t; jj < m; jj += 64) {
+        const zd p = Hs[k * m + jj], q = Hs[(k + 1) * m + jj];
+        Hs[k * m + jj] = p * cg + sg * q;
+        Hs[(k + 1) * m + jj] = q * cg - conj(sg) * p;
+      }
+      __syncthreads();
+      const int rmax = min(k + 2, hi);
+      for (int ii = t; ii <= rmax; ii += 64) {
+        const zd p = Hs[ii * m + k], q = Hs[ii * m + k + 1];
+        Hs[ii * m + k] = p * cg + conj(sg) * q;
+        Hs[ii * m + k + 1] = q * cg - sg * p;
+      }
+      for (int ii = t; ii < m; ii += 64) {
+        const zd p = Q[ii * m + k], q = Q[ii * m + k + 1];
+        Q[ii * m + k] = p * cg + conj(sg) * q;
+        Q[ii * m + k + 1] = q * cg - sg * p;
+      }
+      __syncthreads();
+      if (k > lo && t == 0) Hs[(k + 1) * m + k - 1] = czero<double>();
+      __syncthreads();
+    }
+  }
+  __syncthreads();
+  // --- reorder: bring the `keep` eigenvalues of largest modulus to the front
+  for (int pos = 0; pos < keep; ++pos) {
+    int best = pos;
+    double bv = cabs(Hs[pos * m + pos]);
+    for (int i = pos + 1; i < m; ++i) {
+      const double vv = cabs(Hs[i * m + i]);
+      if (vv > bv) { bv = vv; best = i; }
+    }
+    for (int k = best - 1; k >= pos; --k) {
+      const zd t11 = Hs[k * m + k], t22 = Hs[(k + 1) * m + k + 1];
+      double cg; zd sg;
+      givens(Hs[k * m + k + 1], t22 - t11, cg, sg);
+      __syncthreads();
+      for (int jj = k + t; jj < m; jj += 64) {
+        const zd p = Hs[k * m + jj], q = Hs[(k + 1) * m + jj];
+        Hs[k * m + jj] = p * cg + sg * q;
+        Hs[(k + 1) * m + jj] = q * cg - conj(sg) * p;
+      }
+      __syncthreads();
+      for (int ii = t; ii <= k + 1; ii += 64) {
+        const zd p = Hs[ii * m + k], q = Hs[ii * m + k + 1];
+        Hs[ii * m + k] = p * cg + conj(sg) * q;
+        Hs[ii * m + k + 1] = q * cg - sg * p;
+      }
+      for (int ii = t; ii < m; ii += 64) {
+        const zd p = Q[ii * m + k], q = Q[ii * m + k + 1];
+        Q[ii * m + k] = p * cg + conj(sg) * q;
+        Q[ii * m + k + 1] = q * cg - sg * p;
+      }
+      __syncthreads();
+      if (t == 0) {
+        Hs[k * m + k] = t22;
+        Hs[(k + 1) * m + k + 1] = t11;
+        Hs[(k + 1) * m + k] = czero<double>();
+      }
+      __syncthreads();
+    }
+  }
+  // --- eigenvectors of T for the first nev (lane i handles pair i)
+  if (t < nev) {
+    const int i = t;
+    zd *z = zz + i * m;
+    const zd ti = Hs[i * m + i];
+    const double smin = fmax(eps * cabs(ti), 1e-300);
+    for (int l = 0; l < m; ++l) z[l] = czero<double>();
+    z[i] = mk(1.0, 0.0);
+    for (int l = i - 1; l >= 0; --l) {
+      zd s = czero<double>();
+      for (int c = l + 1; c <= i; ++c) s = cfma(s, Hs[l * m + c], z[c]);
+      zd d = Hs[l * m + l] - ti;
+      if (cabs(d) < smin) d = mk(smin, 0.0);
+      z[l] = czero<double>() - cdiv(s, d);
+    }
+  }
+  __syncthreads();
+  // y_i = Q z_i, normalised; residual estimate |h_{m,m-1}| |y_i[m-1]|
+  __shared__ double s_est[KS_NEVMAX];
+  if (t < nev) {
+    const int i = t;
+    const zd *z = zz + i * m;
+    double nn = 0.0;
+    zd ylast = czero<double>();
+    for (int r = 0; r < m; ++r) {
+      zd s = czero<double>();
+      for (int c = 0; c <= i; ++c) s = cfma(s, Q[r * m + c], z[c]);
+      nn += abs2(s);
+      if (r == m - 1) ylast = s;
+    }
+    const double inv = 1.0 / sqrt(nn);
+    for (int r = 0; r < m; ++r) {
+      zd s = czero<double>();
+      for (int c = 0; c <= i; ++c) s = cfma(s, Q[r * m + c], z[c]);
+      Yg[(size_t)i * KS_MMAX + r] = s * inv;
+    }
+    const zd th = Hs[i * m + i];
+    s_est[i] = cabs(hlast) * cabs(ylast) * inv / fmax(cabs(th), 1e-300);
+  }
+  __syncthreads();
+  // write T (row-major -> column-major global H) and Q, theta
+  for (int e = t; e < m * m; e += 64) {
+    const int i = e % m, j = e / m;
+    Qg[(size_t)j * KS_MMAX + i] = Q[i * m + j];
+  }
+  for (int i = t; i < m; i += 64) THg[i] = Hs[i * m + i];
+  for (int e = t; e < keep * keep; e += 64) {
+    const int i = e % keep, j = e / keep;
+    Hg[(size_t)j * ldh + i] = (i <= j) ? Hs[i * m + j] : czero<double>();
+  }
+  __syncthreads();
+  if (t == 0) {
+    double mx = 0.0;
+    for (int i = 0; i < nev; ++i) mx = fmax(mx, s_est[i]);
+    S.resmax = mx;
+    S.done = (mx <= tol) ? 1 : 0;
+    S.cycles += 1;
+    st[blockIdx.x] = S;
+    // keep hlast for the restart in H(m, m-1) (column m-1, row m) untouched
+  }
+}
+
+// Krylov-Schur truncation to `keep` columns: V <- V Q(:, 0:keep), v_keep <- v_m,
+// H <- [T_keep; b^H], b^H = h_{m,m-1} Q(m-1, 0:keep).
+template <int NT, int KMAX>
+__global__ __launch_bounds__(NT) void k_restart(const PointDev *__restrict__ pts, zd *__restrict__ Vall,
+                                                zd *__restrict__ small, KSState *__restrict__ st, int m, int keep) {
+  const PointDev &P = pts[blockIdx.x];
+  KSState S = st[blockIdx.x];
+  if (S.done) return;
+  __shared__ zd s_q[KS_MMAX * KMAX];
+  const int t = threadIdx.x, nv = P.nvec;
+  const size_t ldv = (size_t)nv;
+  zd *V = Vall + P.vec_off;
+  zd *H = small + P.small_off + SM_H;
+  const zd *Qg = small + P.small_off + SM_Q;
+  const int ldh = KS_MMAX + 1;
+  for (int e = t; e < m * keep; e += NT) {
+    const int r = e % m, c = e / m;
+    s_q[r * KMAX + c] = Qg[(size_t)c * KS_MMAX + r];
+  }
+  const zd hlast = H[(size_t)(m - 1) * ldh + m];
+  __syncthreads();
+  for (int r = t; r < nv; r += NT) {
+    zd acc[KMAX];
+#pragma unroll
+    for (int c = 0; c < KMAX; ++c) acc[c] = czero<double>();
+    for (int l = 0; l < m; ++l) {
+      const zd v = V[(size_t)l * ldv + r];
+#pragma unroll
+      for (int c = 0; c < KMAX; ++c)
+        if (c < keep) acc[c] = cfma(acc[c], v, s_q[l * KMAX + c]);
+    }
+    const zd vm = V[(size_t)m * ldv + r];
+#pragma unroll
+    for (int c = 0; c < KMAX; ++c)
+      if (c < keep) V[(size_t)c * ldv + r] = acc[c];
+    V[(size_t)keep * ldv + r] = vm;
+  }
+  __syncthreads();
+  // rebuild H: T block already written by k_schur in rows/cols < keep
+  for (int e = t; e < ldh * KS_MMAX; e += NT) {
+    const int i = e % ldh, j = e / ldh;
+    if (j >= keep || i > keep) H[e] = czero<double>();
+  }
+  __syncthreads();
+  // b^T = h_{m,m-1} e_{m-1}^T Q(:, 0:keep)
+  for (int j = t; j < keep; j += NT) H[(size_t)j * ldh + keep] = hlast * s_q[(m - 1) * KMAX + j];
+  if (t == 0) { S.j0 = keep; S.k = keep; st[blockIdx.x] = S; }
+}
+
+// Final Ritz pairs: x_i = V y_i, omega_i = sigma + 1/theta_i, getevec-style
+// normalisation (first entry of largest modulus among the first 5 ny
+// components becomes 1, getevec.f90:179-189), FP64 relative residual.
+template <int NT>
+__global__ __launch_bounds__(NT) void k_final(const PointDev *__restrict__ pts, const zd *__restrict__ Vall,
+                                              const zd *__restrict__ small, const zd *__restrict__ coef,
+                                              const double *__restrict__ ops, const double *__restrict__ norms,
+                                              int m, zd *__restrict__ eval, zd *__restrict__ evec,
+                                              double *__restrict__ resid, int npadmax, int nymax) {
+  extern __shared__ zd lds[];
+  const PointDev &P = pts[blockIdx.x];
+  OpLds L(lds, P, npadmax, nymax);
+  const int t = threadIdx.x, nv = P.nvec, n = P.n;
+  const size_t ldv = (size_t)nv;
+  const zd *V = Vall + P.vec_off;
+  const zd *Yg = small + P.small_off + SM_Y;
+  const zd *THg = small + P.small_off + SM_TH;
+  zd *out = evec + P.out_off;
+  __shared__ double s_mx[NT];
+  __shared__ int s_ix[NT];
+  for (int i = 0; i < P.nev; ++i) {
+    const zd th = THg[i];
+    const zd om = P.shift + cdiv(mk(1.0, 0.0), th);
+    for (int r = t; r < nv; r += NT) {
+      zd s = czero<double>();
+      for (int l = 0; l < m; ++l) s = cfma(s, V[(size_t)l * ldv + r], Yg[(size_t)i * KS_MMAX + l]);
+      L.z[r] = s;
+    }
+    __syncthreads();
+    // first entry of largest modulus among the first n components
+    double bm = -1.0; int bi = 0x7fffffff;
+    for (int r = t; r < n; r += NT) { const double a = cabs(L.z[r]); if (a > bm) { bm = a; bi = r; } }
+    s_mx[t] = bm; s_ix[t] = bi;
+    __syncthreads();
+    if (t == 0) {
+      for (int k = 1; k < NT; ++k)
+        if (s_mx[k] > bm || (s_mx[k] == bm && s_ix[k] < bi)) { bm = s_mx[k]; bi = s_ix[k]; }
+      s_ix[0] = bi;
+    }
+    __syncthreads();
+    const zd scale = L.z[s_ix[0]];
+    __syncthreads();
+    for (int r = t; r < nv; r += NT) L.z[r] = cdiv(L.z[r], scale);
+    __syncthreads();
+    // FP64 residual at the Ritz value and at the Rayleigh-quotient value of the
+    // same vector; the smaller one is kept.  An operator error along the wanted
+    // eigenvector (the near-null direction of A - sigma B when the shift is
+    // close) moves only the Ritz value, not the vector, so the quotient with
+    // the exact pencil recovers the eigenvalue.
+    double rn, xn, den;
+    zd om_out = om;
+    if (P.itype == 1) {
+      pencil_apply<NT>(P, coef, ops, FORM_AM, P.shift, L.z, L.y, L.X1, L.X2, L.red);
+      pencil_apply<NT>(P, coef, ops, FORM_BM, P.shift, L.z, L.rhs, L.X1, L.X2, L.red);
+      zd d3[3] = {czero<double>(), czero<double>(), czero<double>()};
+      for (int r = t; r < n; r += NT) {
+        d3[0] = cfmac(d3[0], L.z[r], L.y[r]);
+        d3[1] = cfmac(d3[1], L.z[r], L.rhs[r]);
+      }
+      block_reduce<NT, 3>(d3, L.red);
+      const zd om_rq = (cabs(d3[1]) > 0.0) ? cdiv(d3[0], d3[1]) : om;
+      zd part[3] = {czero<double>(), czero<double>(), czero<double>()};
+      for (int r = t; r < n; r += NT) {
+        part[0].x += abs2(L.y[r] - om * L.rhs[r]);
+        part[1].x += abs2(L.y[r] - om_rq * L.rhs[r]);
+        part[2].x += abs2(L.z[r]);
+      }
+      block_reduce<NT, 3>(part, L.red);
+      const bool use_rq = part[1].x < part[0].x;
+      om_out = use_rq ? om_rq : om;
+      rn = sqrt(use_rq ? part[1].x : part[0].x);
+      xn = sqrt(part[2].x);
+      den = sqrt(norms[blockIdx.x * 4 + 0]) + cabs(om_out) * sqrt(norms[blockIdx.x * 4 + 1]);
+    } else {
+      // C0 phi -> y, C1 phi -> rhs, C2 phi -> res
+      pencil_apply<NT>(P, coef, ops, FORM_C0, P.shift, L.z, L.y, L.X1, L.X2, L.red);
+      pencil_apply<NT>(P, coef, ops, FORM_C1, P.shift, L.z, L.rhs, L.X1, L.X2, L.red);
+      pencil_apply<NT>(P, coef, ops, FORM_C2, P.shift, L.z, L.res, L.X1, L.X2, L.red);
+      zd d3[3] = {czero<double>(), czero<double>(), czero<double>()};
+      for (int r = t; r < n; r += NT) {
+        d3[0] = cfmac(d3[0], L.z[r], L.y[r]);
+        d3[1] = cfmac(d3[1], L.z[r], L.rhs[r]);
+        d3[2] = cfmac(d3[2], L.z[r], L.res[r]);
+      }
+      block_reduce<NT, 3>(d3, L.red);
+      // roots of c2 a^2 + c1 a + c0 = 0 nearest the Ritz value
+      zd om_rq = om;
+      if (cabs(d3[2]) > 0.0) {
+        const zd bb = cdiv(d3[1], d3[2] * 2.0), cc = cdiv(d3[0], d3[2]);
+        zd disc = bb * bb - cc;
+        const double rr = cabs(disc);
+        const double sr = sqrt(0.5 * (rr + fabs(disc.x)));
+        zd sq;
+        if (sr == 0.0) sq = czero<double>();
+        else if (disc.x >= 0) sq = mk(sr, disc.y / (2.0 * sr));
+        else sq = mk(fabs(disc.y) / (2.0 * sr), disc.y >= 0 ? sr : -sr);
+        const zd r1 = czero<double>() - bb + sq, r2 = czero<double>() - bb - sq;
+        om_rq = (cabs(r1 - om) < cabs(r2 - om)) ? r1 : r2;
+      } else if (cabs(d3[1]) > 0.0) {
+        om_rq = czero<double>() - cdiv(d3[0], d3[1]);
+      }
+      zd part[3] = {czero<double>(), czero<double>(), czero<double>()};
+      for (int r = t; r < n; r += NT) {
+        part[0].x += abs2(L.y[r] + om * L.rhs[r] + (om * om) * L.res[r]);
+        part[1].x += abs2(L.y[r] + om_rq * L.rhs[r] + (om_rq * om_rq) * L.res[r]);
+        part[2].x += abs2(L.z[r]);
+      }
+      block_reduce<NT, 3>(part, L.red);
+      const bool use_rq = part[1].x < part[0].x;
+      om_out = use_rq ? om_rq : om;
+      rn = sqrt(use_rq ? part[1].x : part[0].x);
+      xn = sqrt(part[2].x);
+      const double a = cabs(om_out);
+      den = sqrt(norms[blockIdx.x * 4 + 0]) + a * sqrt(norms[blockIdx.x * 4 + 1]) + a * a * sqrt(norms[blockIdx.x * 4 + 2]);
+    }
+    if (t == 0) {
+      eval[(size_t)blockIdx.x * KS_NEVMAX + i] = om_out;
+      resid[(size_t)blockIdx.x * KS_NEVMAX + i] = rn / (den * xn);
+    }
+    for (int r = t; r < nv; r += NT) out[(size_t)i * nv + r] = L.z[r];
+    __syncthreads();
+  }
+}
+
+// One shifted solve M x = b (factors from k_lu, `ir` refinement steps); used by
+// the stab_shifted_solve entry point.
+template <typename T, int NB, int NT>
+__global__ __launch_bounds__(NT) void k_solve_one(const PointDev *__restrict__ pts, const cx<T> *__restrict__ mats,
+                                                  const int *__restrict__ pivs, const cx<T> *__restrict__ blks,
+                                                  const double *__restrict__ scales, const zd *__restrict__ coef,
+                                                  const double *__restrict__ ops, const zd *__restrict__ b,
+                                                  zd *__restrict__ x, int ir, int npadmax, int nymax) {
+  extern __shared__ zd lds[];
+  const PointDev &P = pts[blockIdx.x];
+  OpLds L(lds, P, npadmax, nymax);
+  const cx<T> *A = mats + P.mat_off;
+  const int *ipiv = pivs + P.piv_off;
+  const cx<T> *Lb = blks + P.blk_off, *Ub = Lb + (size_t)P.npad * NB;
+  const double *dr = scales + P.scale_off, *dc = dr + P.npad;
+  for (int r = threadIdx.x; r < P.n; r += NT) L.rhs[r] = b[r];
+  __syncthreads();
+  shifted_solve<T, NB, NT>(P, A, ipiv, Lb, Ub, dr, dc, coef, ops, ir, L);
+  for (int r = threadIdx.x; r < P.n; r += NT) x[r] = L.y[r];
+}
