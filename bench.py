@@ -10,11 +10,12 @@ sigma = 0.33 alpha (a phase-speed guess, as a getevec -v guess) and asks for
 the 8 eigenpairs nearest it.  A mode counts as converged when its FP64
 relative residual ||(A - w B) x|| / ((||A|| + |w| ||B||) ||x||) < 1e-10.
 
-value = converged modes per second of the chosen path (FP32 LU with FP64
-refinement); the plain FP64 path is timed on the same batch and reported as
-``plain``; the CPU baseline times the oracle (ZGESV + ZGEEV, the reference's
-method) on a bounded sample and checks the GPU eigenvalues against ZGGEV on
-the same points.
+value = converged modes per second of the faster GPU path on this workload.
+Both paths are timed on the same batch: ``plain`` (FP64 LU + FP64 shift-invert
+Krylov-Schur) and ``mixed`` (equilibrated FP32 LU, FP64 iterative refinement
+inside every operator application, FP64 fallback per point).  The CPU baseline
+times the oracle (ZGESV + ZGEEV, the reference's method) on a bounded sample
+and checks the GPU eigenvalues against ZGGEV on the same points.
 """
 from __future__ import annotations
 
@@ -63,7 +64,8 @@ def cpu_worker(args):
     os.environ["OPENBLAS_NUM_THREADS"] = "1"
     tab, a, b, r, n = args
     from oracle import stab_oracle as so
-    txt = f"1\n288.15\n0.5 {r!r} 0.72\n{int(n)} {2.0 / 46.0!r} 50\n1\n1\n0 0 0 0\n1\n{a!r} 0\n{b!r} 0\n0\n"
+    txt = (f"1\n288.15\n0.5 {float(r)!r} 0.72\n{int(n)} {2.0 / 46.0!r} 50\n1\n1\n0 0 0 0\n1\n"
+           f"{float(a)!r} 0\n{float(b)!r} 0\n0\n")
     p = so.parse_input(txt)
     t0 = time.perf_counter()
     pb = so.setup(p, tab[:, 0], tab[:, 1:6].copy(), spatial=False)
@@ -106,7 +108,7 @@ def main():
     ap.add_argument("--nev", type=int, default=8)
     ap.add_argument("--cpu-sample", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
@@ -132,26 +134,35 @@ def main():
             stats.append(res.stats)
         return res, times, stats
 
-    print(f"# bench: {args.points} points, mode mixed", file=sys.stderr, flush=True)
-    res, times, stats = run(_lib.MODE_MIXED, args.steps, args.warmup)
-    ok = (res.resid[:, :args.nev] < 1e-10) & (res.info[:, None] == 0)
-    nconv = int(ok.sum())
-    t_step = float(np.mean(times))
-    value = nconv / t_step
-    st = stats[-1]
-    ms_lu = float(np.mean([s["ms_lu"] for s in stats]))
-    ms_eig = float(np.mean([s["ms_eig"] for s in stats]))
-    lu_tflops = st["lu_flops"] / (ms_lu * 1e-3) / 1e12
+    def summarize(res, times, stats):
+        ok = (res.resid[:, :args.nev] < 1e-10) & (res.info[:, None] == 0)
+        nconv = int(ok.sum())
+        t = float(np.mean(times))
+        ms_arn = float(np.mean([s["ms_arnoldi"] for s in stats]))
+        sbytes = float(np.mean([s["solve_bytes"] for s in stats]))
+        ms_lu = float(np.mean([s["ms_lu"] for s in stats]))
+        return dict(value=nconv / t, ms_per_step=t * 1e3, converged=nconv, ms_lu=ms_lu,
+                    ms_eig=float(np.mean([s["ms_eig"] for s in stats])), ms_arnoldi=ms_arn,
+                    arnoldi_gbs=sbytes / (ms_arn * 1e-3) / 1e9 if ms_arn > 0 else None,
+                    lu_tflops=stats[-1]["lu_flops"] / (ms_lu * 1e-3) / 1e12,
+                    ks_cycles_max=stats[-1]["restarts_max"], nfallback=stats[-1]["nfallback"],
+                    op_applies=stats[-1]["op_applies"])
 
-    plain = None
-    if not args.no_plain:
-        print("# bench: plain FP64 path", file=sys.stderr, flush=True)
-        pres, ptimes, pstats = run(_lib.MODE_PLAIN_F64, 1, 0)
-        pok = (pres.resid[:, :args.nev] < 1e-10) & (pres.info[:, None] == 0)
-        pt = float(np.mean(ptimes))
-        plain = dict(value=int(pok.sum()) / pt, ms_per_step=pt * 1e3, converged=int(pok.sum()),
-                     ms_lu=pstats[-1]["ms_lu"], ms_eig=pstats[-1]["ms_eig"],
-                     lu_tflops=pstats[-1]["lu_flops"] / (pstats[-1]["ms_lu"] * 1e-3) / 1e12)
+    # production path for the 8-mode sweep: FP64 LU + FP64 shift-invert Krylov-Schur
+    print(f"# bench: {args.points} points, plain FP64 path", file=sys.stderr, flush=True)
+    res, times, stats = run(_lib.MODE_PLAIN_F64, args.steps, args.warmup)
+    plain = summarize(res, times, stats)
+    print(f"# plain: {plain}", file=sys.stderr, flush=True)
+
+    mixed = None
+    if not args.no_mixed:
+        print("# bench: FP32-LU + FP64-refinement path (1 step)", file=sys.stderr, flush=True)
+        mres, mtimes, mstats = run(_lib.MODE_MIXED, 1, 0)
+        mixed = summarize(mres, mtimes, mstats)
+        print(f"# mixed: {mixed}", file=sys.stderr, flush=True)
+
+    best = plain if (mixed is None or plain["value"] >= mixed["value"]) else mixed
+    best_name = "plain FP64" if best is plain else "FP32 LU + FP64 refinement"
 
     cpu = None
     if not args.no_cpu:
@@ -162,32 +173,35 @@ def main():
                "sample": f"first {args.cpu_sample} points of the same seeded sweep, reference method "
                          f"(ZGESV+ZGEEV full spectrum, 8 modes counted per point), one point per process, "
                          f"OPENBLAS_NUM_THREADS=1; {cb['per_point_s']:.2f} s per point on one core",
-               "qz_match": f"{cb['match']}/{cb['total']}"}
+               "qz_match": f"{cb['match']}/{cb['total']} GPU eigenvalues within 1e-8 of a ZGGEV eigenvalue"}
 
     out = {
-        "metric": "leading eigenmodes/sec at |dw|<=1e-8 (FP32 LU + FP64 refinement path)",
-        "value": value,
+        "metric": "leading eigenmodes/sec at |dw|<=1e-8 (8 eigenpairs nearest the input shift, FP64 residual < 1e-10)",
+        "value": best["value"],
         "unit": "leading eigenmodes/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": t_step * 1e3,
+        "ms_per_step": best["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c64 LU / f64 refinement",
+        "dtype": "c128 (f64)" if best is plain else "c64 LU / f64 refinement",
         "data": "synthetic (seeded parameter sweep over a computed compressible Blasius profile)",
-        "config": {"workload": "temporal sweep, compressible Blasius M=0.5, 4096 points, N in {64,96,128}, "
-                               "8 modes nearest sigma=0.33 alpha",
-                   "points": args.points, "nev": args.nev, "seed": args.seed,
-                   "converged_modes": nconv, "total_modes": int(args.points * args.nev),
-                   "ms_lu": ms_lu, "ms_eig": ms_eig, "ms_prep": st["ms_prep"], "ms_assemble": st["ms_assemble"],
-                   "ks_cycles_max": st["restarts_max"]},
+        "config": {"workload": f"temporal sweep, compressible Blasius M=0.5, {args.points} points, "
+                               "N in {64,96,128}, 8 modes nearest sigma=0.33 alpha",
+                   "path": best_name, "points": args.points, "nev": args.nev, "seed": args.seed,
+                   "converged_modes": best["converged"], "total_modes": int(args.points * args.nev)},
         "plain": plain,
-        "vs_plain": (value / plain["value"]) if plain and plain["value"] > 0 else None,
-        "roofline": {"bound": "mfma", "kernel": "k_lu (batched complex LU, FP32)", "achieved": lu_tflops,
-                     "peak": PEAK_FP32_MFMA_TFLOPS, "unit": "TFLOP/s", "frac": lu_tflops / PEAK_FP32_MFMA_TFLOPS,
-                     "traffic": None},
+        "mixed": mixed,
+        "vs_plain": best["value"] / plain["value"],
+        "roofline": {"bound": "hbm", "kernel": "k_arnoldi (shift-invert Krylov expansion: blocked triangular "
+                                               "solves + CGS2)",
+                     "achieved": plain["arnoldi_gbs"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": (plain["arnoldi_gbs"] or 0.0) / PEAK_HBM_GBS, "traffic": None,
+                     "algorithmic_bytes": "16 npad^2 per operator application (one pass over the FP64 LU factors)"},
+        "lu": {"kernel": "k_lu (batched complex LU, FP64)", "achieved_tflops": plain["lu_tflops"],
+               "peak_tflops": PEAK_FP64_MFMA_TFLOPS, "frac": plain["lu_tflops"] / PEAK_FP64_MFMA_TFLOPS},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

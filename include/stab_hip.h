@@ -45,7 +45,8 @@ typedef struct { double re, im; } stab_z;
 
 /* Solve modes */
 #define STAB_MODE_PLAIN_F64  0   /* FP64 LU + FP64 shift-invert Arnoldi */
-#define STAB_MODE_MIXED      1   /* FP32 LU + FP64-residual refined eigenpairs */
+#define STAB_MODE_MIXED      1   /* equilibrated FP32 LU + FP64 iterative refinement; points whose
+                                      FP32 factors contract poorly are re-solved in FP64 */
 
 /* Matrices returned by stab_pencil (reference form, for parity checks) */
 #define STAB_MAT_A0  0   /* temporal A0  (temporal.f90:664-752) */
@@ -96,7 +97,11 @@ typedef struct {
   int64_t lu_flops;       /* algorithmic (8/3) n^3 per point, summed */
   int64_t assemble_bytes; /* algorithmic bytes written/read by assembly */
   int32_t npts, nconv;    /* points, points whose nev pairs all converged */
-  int32_t restarts_max;
+  int32_t restarts_max;   /* Krylov-Schur cycles of the slowest point */
+  int32_t nfallback;      /* STAB_MODE_MIXED: points re-solved by the FP64 path */
+  double ms_arnoldi;      /* device time of the Arnoldi-expansion kernel launches */
+  int64_t op_applies;     /* shift-invert operator applications, summed over points */
+  double solve_bytes;     /* algorithmic factor bytes read by those applications */
 } stab_stats;
 
 /* Context on HIP device `device`.  New in this library (the reference keeps
@@ -152,6 +157,16 @@ int stab_solve(stab_ctx *ctx, int npts, const stab_point *pts, int mode, stab_z 
  * exact zero pivot. */
 int stab_shifted_solve(stab_ctx *ctx, const stab_point *pt, int mode, int ir_steps, const stab_z *b, stab_z *x,
                        int32_t *info);
+
+/* LU factors of the shifted matrix of one point as stab_solve computes them:
+ * lu is npad x npad column-major (npad = 5 ny rounded up to a multiple of 32;
+ * rows/columns beyond 5 ny hold the identity), unit-lower L below the
+ * diagonal and U on and above it; perm[r] is the input row that ends in row r
+ * (P A = L U).  STAB_MODE_MIXED factors Dr A Dc in FP32 and returns the row and
+ * column scales in scales[0:npad] and scales[npad:2 npad] (scales may be NULL).
+ * Replaces ZGETRF (spatial.f90:978). */
+int stab_lu_factors(stab_ctx *ctx, const stab_point *pt, int mode, stab_z *lu, int32_t *perm, double *scales,
+                    int32_t *info);
 
 /* Size in bytes of the evec buffer stab_solve needs for these points. */
 size_t stab_evec_bytes(int npts, const stab_point *pts);

@@ -19,7 +19,7 @@ MAT_A0, MAT_B0, MAT_C0, MAT_C1, MAT_C2, MAT_SHIFTED = range(6)
 INFO_OK, INFO_SINGULAR, INFO_NOCONV = 0, 1, 2
 
 EXPORTS = ("stab_ctx_create", "stab_ctx_destroy", "stab_last_error", "stab_profile_set", "stab_grid",
-           "stab_pencil", "stab_solve", "stab_evec_bytes", "stab_shifted_solve")
+           "stab_pencil", "stab_solve", "stab_evec_bytes", "stab_shifted_solve", "stab_lu_factors")
 
 
 class Z(C.Structure):
@@ -39,7 +39,8 @@ class Stats(C.Structure):
     _fields_ = [("ms_prep", C.c_double), ("ms_assemble", C.c_double), ("ms_lu", C.c_double),
                 ("ms_eig", C.c_double), ("ms_total", C.c_double), ("lu_flops", C.c_int64),
                 ("assemble_bytes", C.c_int64), ("npts", C.c_int32), ("nconv", C.c_int32),
-                ("restarts_max", C.c_int32)]
+                ("restarts_max", C.c_int32), ("nfallback", C.c_int32), ("ms_arnoldi", C.c_double),
+                ("op_applies", C.c_int64), ("solve_bytes", C.c_double)]
 
 
 _lib = None
@@ -65,6 +66,7 @@ def load():
                                C.POINTER(Stats)]
     lib.stab_evec_bytes.argtypes = [C.c_int, C.POINTER(Point)]
     lib.stab_shifted_solve.argtypes = [vp, C.POINTER(Point), C.c_int, C.c_int, C.POINTER(Z), C.POINTER(Z), ip]
+    lib.stab_lu_factors.argtypes = [vp, C.POINTER(Point), C.c_int, C.POINTER(Z), ip, dp, ip]
     lib.stab_evec_bytes.restype = C.c_size_t
     for name in EXPORTS:
         lib[name].restype = lib[name].restype if name in ("stab_last_error", "stab_evec_bytes") else C.c_int

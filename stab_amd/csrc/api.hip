@@ -185,13 +185,14 @@ int build_batch(stab_ctx *c, int npts, const stab_point *pts, size_t elem, int m
     d.mat_off = moff; moff += (long long)d.npad * d.npad;
     d.vec_off = voff; voff += (long long)d.nvec * (m + 1);
     d.small_off = (long long)i * SM_SIZE;
-    d.piv_off = pvoff; pvoff += d.npad;
-    d.blk_off = boff; boff += 2LL * d.npad * 32;
+    d.piv_off = pvoff; pvoff += 2LL * d.npad;   // ipiv then perm
+    d.blk_off = boff; boff += blk_elems(d.npad, SOLVE_BS);
     d.scale_off = soff; soff += 2LL * d.npad;
     d.out_off = eoff; eoff += (long long)d.nvec * STAB_MAX_NEV;
     B.npadmax = std::max(B.npadmax, d.npad);
     B.nymax = std::max(B.nymax, d.ny);
-    B.nvecmax = std::max(B.nvecmax, d.nvec);
+    // z also serves as the npad-long gather buffer of the compact solve
+    B.nvecmax = std::max(B.nvecmax, std::max(d.nvec, d.npad));
   }
   B.nscales = (size_t)soff;
   int rc;
@@ -265,8 +266,12 @@ int launch_lu_nt(stab_ctx *c, int npts, Batch &B) {
 
 template <typename T, int NB>
 int launch_lu(stab_ctx *c, int npts, Batch &B) {
-  if (B.npadmax <= 768) return launch_lu_nt<T, NB, 256>(c, npts, B);
-  return launch_lu_nt<T, NB, 512>(c, npts, B);
+  int rc = (B.npadmax <= 768) ? launch_lu_nt<T, NB, 256>(c, npts, B) : launch_lu_nt<T, NB, 512>(c, npts, B);
+  if (rc != STAB_OK) return rc;
+  hipLaunchKernelGGL((k_blkinv<T, SOLVE_BS>), dim3((B.npadmax + SOLVE_BS - 1) / SOLVE_BS, npts), dim3(SOLVE_BS), 0,
+                     c->stream, B.pts, (const cx<T> *)B.mats, (cx<T> *)B.blks);
+  HIPCHECK(c, hipGetLastError());
+  return STAB_OK;
 }
 
 struct KSParams {
@@ -275,35 +280,52 @@ struct KSParams {
 };
 
 template <typename T, int NB>
-int run_eig(stab_ctx *c, int npts, Batch &B, const KSParams &kp, int &cycles_max, int &nconv) {
-  const size_t lds = op_lds_bytes(B.npadmax, B.nymax);
+int run_eig(stab_ctx *c, int npts, Batch &B, const KSParams &kp, int &cycles_max, int &nconv, double &ms_arnoldi) {
+  bool any_spatial = false;
+  for (const auto &d : B.h) any_spatial |= (d.itype == 2);
+  const int full = (kp.ir != 0 || any_spatial) ? 1 : 0;
+  const size_t lds = op_lds_bytes(B.npadmax, B.nymax, 1, B.nvecmax);
+  const size_t lds_arn = op_lds_bytes(B.npadmax, B.nymax, full, B.nvecmax);
   if (lds > 160 * 1024) return fail(c, STAB_E_ARG, "operator workspace %zu B exceeds LDS", lds);
   HIPCHECK(c, hipMemsetAsync(B.state, 0, sizeof(KSState) * npts, c->stream));
   const cx<T> *mats = (const cx<T> *)B.mats, *blks = (const cx<T> *)B.blks;
   const size_t lds_schur = sizeof(zd) * (2 * kp.m * kp.m + kp.m + KS_NEVMAX * kp.m);
   std::vector<KSState> hs(npts);
   cycles_max = 0;
+  ms_arnoldi = 0.0;
   for (int cyc = 0; cyc < kp.maxcycles; ++cyc) {
-    hipLaunchKernelGGL((k_arnoldi<T, NB, 256>), dim3(npts), dim3(256), lds, c->stream, B.pts, mats, B.pivs, blks,
-                       B.scales, B.coef, B.ops, B.V, B.small, B.state, kp.m, kp.ir, B.npadmax, B.nymax,
-                       cyc == 0 ? 1 : 0);
+    HIPCHECK(c, hipEventRecord(c->ev[5], c->stream));
+    if (full)
+      hipLaunchKernelGGL((k_arnoldi<T, NB, 256, true>), dim3(npts), dim3(256), lds_arn, c->stream, B.pts, mats,
+                         B.pivs, blks, B.scales, B.coef, B.ops, B.V, B.small, B.state, kp.m, kp.ir, B.npadmax,
+                         B.nymax, B.nvecmax, cyc == 0 ? 1 : 0);
+    else
+      hipLaunchKernelGGL((k_arnoldi<T, NB, 256, false>), dim3(npts), dim3(256), lds_arn, c->stream, B.pts, mats,
+                         B.pivs, blks, B.scales, B.coef, B.ops, B.V, B.small, B.state, kp.m, kp.ir, B.npadmax,
+                         B.nymax, B.nvecmax, cyc == 0 ? 1 : 0);
+    HIPCHECK(c, hipEventRecord(c->ev[6], c->stream));
     hipLaunchKernelGGL(k_schur, dim3(npts), dim3(64), lds_schur, c->stream, B.pts, B.small, B.state, kp.m, kp.keep,
                        kp.tol);
     HIPCHECK(c, hipGetLastError());
     HIPCHECK(c, hipMemcpyAsync(hs.data(), B.state, sizeof(KSState) * npts, hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(c, hipStreamSynchronize(c->stream));
+    {
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, c->ev[5], c->ev[6]);
+      ms_arnoldi += ms;
+    }
     cycles_max = cyc + 1;
     int ndone = 0;
-    for (auto &s : hs) ndone += s.done;
+    for (auto &s : hs) ndone += (s.done != 0);
     if (ndone == npts || cyc == kp.maxcycles - 1) break;
     hipLaunchKernelGGL((k_restart<256, 32>), dim3(npts), dim3(256), 0, c->stream, B.pts, B.V, B.small, B.state,
                        kp.m, kp.keep);
     HIPCHECK(c, hipGetLastError());
   }
   nconv = 0;
-  for (auto &s : hs) nconv += s.done;
+  for (auto &s : hs) nconv += (s.done == 1);
   hipLaunchKernelGGL(k_final<256>, dim3(npts), dim3(256), lds, c->stream, B.pts, B.V, B.small, B.coef, B.ops, B.norms,
-                     kp.m, B.eval, B.evec, B.resid, B.npadmax, B.nymax);
+                     kp.m, B.eval, B.evec, B.resid, B.npadmax, B.nymax, B.nvecmax);
   HIPCHECK(c, hipGetLastError());
   return STAB_OK;
 }
@@ -469,7 +491,7 @@ int stab_shifted_solve(stab_ctx *c, const stab_point *pt, int mode, int ir_steps
   if ((rc = buf_get(c, "solve_io", sizeof(zd) * 2 * (size_t)n, &p)) != STAB_OK) return rc;
   zd *db = (zd *)p, *dx = db + n;
   HIPCHECK(c, hipMemcpyAsync(db, b, sizeof(zd) * n, hipMemcpyHostToDevice, c->stream));
-  const size_t lds = op_lds_bytes(B.npadmax, B.nymax);
+  const size_t lds = op_lds_bytes(B.npadmax, B.nymax, 1, B.npadmax);
   if (f32)
     hipLaunchKernelGGL((k_solve_one<float, 32, 256>), dim3(1), dim3(256), lds, c->stream, B.pts, (const zf *)B.mats,
                        B.pivs, (const zf *)B.blks, B.scales, B.coef, B.ops, db, dx, ir_steps, B.npadmax, B.nymax);
@@ -485,14 +507,55 @@ int stab_shifted_solve(stab_ctx *c, const stab_point *pt, int mode, int ir_steps
   return STAB_OK;
 }
 
+int stab_lu_factors(stab_ctx *c, const stab_point *pt, int mode, stab_z *lu, int32_t *perm, double *scales,
+                    int32_t *info) {
+  if (!c || !pt || !lu || !perm) return STAB_E_ARG;
+  if (mode != STAB_MODE_PLAIN_F64 && mode != STAB_MODE_MIXED) return fail(c, STAB_E_ARG, "unknown mode %d", mode);
+  int rc = check_point(c, *pt, 0);
+  if (rc) return rc;
+  (void)hipSetDevice(c->device);
+  const bool f32 = (mode == STAB_MODE_MIXED);
+  Batch B;
+  if ((rc = build_batch(c, 1, pt, f32 ? sizeof(zf) : sizeof(zd), 1, B)) != STAB_OK) return rc;
+  if ((rc = run_prep(c, 1, B, f32)) != STAB_OK) return rc;
+  const int cb = (B.npadmax + 7) / 8;
+  if (f32)
+    hipLaunchKernelGGL((k_assemble<float, 8>), dim3(cb, 1), dim3(256), 0, c->stream, B.pts, B.coef, B.ops, B.scales,
+                       (zf *)B.mats, (int)FORM_SHIFTED, 1);
+  else
+    hipLaunchKernelGGL((k_assemble<double, 8>), dim3(cb, 1), dim3(256), 0, c->stream, B.pts, B.coef, B.ops,
+                       B.scales, (zd *)B.mats, (int)FORM_SHIFTED, 0);
+  if ((rc = f32 ? launch_lu<float, 32>(c, 1, B) : launch_lu<double, 16>(c, 1, B)) != STAB_OK) return rc;
+  const int np = B.h[0].npad;
+  std::vector<int> pv(2 * (size_t)np);
+  std::vector<double> sc(2 * (size_t)np);
+  int inf = 0;
+  HIPCHECK(c, hipMemcpyAsync(pv.data(), B.pivs, sizeof(int) * pv.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(sc.data(), B.scales, sizeof(double) * sc.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(&inf, B.info, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  if (f32) {
+    std::vector<zf> M((size_t)np * np);
+    HIPCHECK(c, hipMemcpyAsync(M.data(), B.mats, sizeof(zf) * M.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    for (size_t e = 0; e < M.size(); ++e) { lu[e].re = M[e].x; lu[e].im = M[e].y; }
+  } else {
+    HIPCHECK(c, hipMemcpyAsync(lu, B.mats, sizeof(zd) * (size_t)np * np, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+  }
+  for (int r = 0; r < np; ++r) perm[r] = pv[np + r];
+  if (scales) memcpy(scales, sc.data(), sizeof(double) * 2 * np);
+  if (info) *info = inf ? STAB_INFO_SINGULAR : STAB_INFO_OK;
+  return STAB_OK;
+}
+
 size_t stab_evec_bytes(int npts, const stab_point *pts) {
   size_t s = 0;
   for (int i = 0; i < npts; ++i) s += (size_t)(pts[i].itype == 1 ? 5 : 10) * pts[i].ny * STAB_MAX_NEV;
   return s * sizeof(stab_z);
 }
 
-int stab_solve(stab_ctx *c, int npts, const stab_point *pts, int mode, stab_z *eval, stab_z *evec, double *resid,
-               int32_t *info, stab_stats *stats) {
+static int solve_impl(stab_ctx *c, int npts, const stab_point *pts, int mode, stab_z *eval, stab_z *evec,
+                      double *resid, int32_t *info, stab_stats *stats, std::vector<int> *irbad) {
   if (!c) return STAB_E_ARG;
   if (npts <= 0 || !pts || !eval || !info) return fail(c, STAB_E_ARG, "npts/pts/eval/info");
   if (mode != STAB_MODE_PLAIN_F64 && mode != STAB_MODE_MIXED) return fail(c, STAB_E_ARG, "unknown mode %d", mode);
@@ -504,7 +567,7 @@ int stab_solve(stab_ctx *c, int npts, const stab_point *pts, int mode, stab_z *e
   KSParams kp;
   kp.m = env_int("STAB_KS_M", 40);
   kp.keep = env_int("STAB_KS_KEEP", 20);
-  kp.maxcycles = env_int("STAB_KS_CYCLES", 12);
+  kp.maxcycles = env_int("STAB_KS_CYCLES", 10);
   kp.ir = (mode == STAB_MODE_MIXED) ? -env_int("STAB_IR_MAX", 12) : 0;   // adaptive refinement
   kp.tol = 1e-12;
   int nevmax = 0;
@@ -530,8 +593,9 @@ int stab_solve(stab_ctx *c, int npts, const stab_point *pts, int mode, stab_z *e
   if ((rc = f32 ? launch_lu<float, 32>(c, npts, B) : launch_lu<double, 16>(c, npts, B)) != STAB_OK) return rc;
   HIPCHECK(c, hipEventRecord(c->ev[3], c->stream));
   int cycles = 0, nconv = 0;
-  if ((rc = f32 ? run_eig<float, 32>(c, npts, B, kp, cycles, nconv) : run_eig<double, 16>(c, npts, B, kp, cycles, nconv)) !=
-      STAB_OK)
+  double ms_arn = 0.0;
+  if ((rc = f32 ? run_eig<float, 32>(c, npts, B, kp, cycles, nconv, ms_arn)
+                : run_eig<double, 16>(c, npts, B, kp, cycles, nconv, ms_arn)) != STAB_OK)
     return rc;
   HIPCHECK(c, hipEventRecord(c->ev[4], c->stream));
   std::vector<zd> ev((size_t)KS_NEVMAX * npts);
@@ -556,7 +620,8 @@ int stab_solve(stab_ctx *c, int npts, const stab_point *pts, int mode, stab_z *e
       eval[e].im = valid ? ev[e].y : 0.0;
       if (resid) resid[e] = valid ? rs[e] : 0.0;
     }
-    info[i] = inf[i] ? STAB_INFO_SINGULAR : (hs[i].done ? STAB_INFO_OK : STAB_INFO_NOCONV);
+    info[i] = inf[i] ? STAB_INFO_SINGULAR : (hs[i].done == 1 ? STAB_INFO_OK : STAB_INFO_NOCONV);
+    if (irbad) (*irbad)[i] = hs[i].irbad || inf[i];
   }
   if (stats) {
     memset(stats, 0, sizeof *stats);
@@ -573,6 +638,63 @@ int stab_solve(stab_ctx *c, int npts, const stab_point *pts, int mode, stab_z *e
     stats->npts = npts;
     stats->nconv = nconv;
     stats->restarts_max = cycles;
+    stats->ms_arnoldi = ms_arn;
+    for (int i = 0; i < npts; ++i) {
+      stats->op_applies += hs[i].steps;
+      // factor bytes read per application: one pass over L and U
+      stats->solve_bytes += (double)hs[i].steps * (f32 ? 8.0 : 16.0) * B.h[i].npad * B.h[i].npad;
+    }
+  }
+  return STAB_OK;
+}
+
+int stab_solve(stab_ctx *c, int npts, const stab_point *pts, int mode, stab_z *eval, stab_z *evec, double *resid,
+               int32_t *info, stab_stats *stats) {
+  std::vector<int> bad(npts > 0 ? npts : 0, 0);
+  int rc = solve_impl(c, npts, pts, mode, eval, evec, resid, info, stats, &bad);
+  if (rc != STAB_OK || mode != STAB_MODE_MIXED) return rc;
+  // Residual-gated fallback: points whose FP32 factors were inadequate for
+  // their shift (refinement contracted poorly, or a zero FP32 pivot) are
+  // solved again by the plain FP64 path.
+  std::vector<int> idx;
+  for (int i = 0; i < npts; ++i)
+    if (bad[i]) idx.push_back(i);
+  if (stats) stats->nfallback = (int32_t)idx.size();
+  if (idx.empty()) return STAB_OK;
+  const int nf = (int)idx.size();
+  std::vector<stab_point> sub(nf);
+  for (int k = 0; k < nf; ++k) sub[k] = pts[idx[k]];
+  std::vector<stab_z> ev((size_t)nf * STAB_MAX_NEV);
+  std::vector<double> rs((size_t)nf * STAB_MAX_NEV);
+  std::vector<int32_t> inf(nf);
+  std::vector<stab_z> vec;
+  if (evec) vec.resize(stab_evec_bytes(nf, sub.data()) / sizeof(stab_z));
+  stab_stats fst;
+  rc = solve_impl(c, nf, sub.data(), STAB_MODE_PLAIN_F64, ev.data(), evec ? vec.data() : nullptr, rs.data(),
+                  inf.data(), &fst, nullptr);
+  if (rc != STAB_OK) return rc;
+  // offsets of each point's vector block in the caller's layout
+  std::vector<size_t> off(npts + 1, 0);
+  for (int i = 0; i < npts; ++i) off[i + 1] = off[i] + (size_t)(pts[i].itype == 1 ? 5 : 10) * pts[i].ny * STAB_MAX_NEV;
+  size_t so = 0;
+  for (int k = 0; k < nf; ++k) {
+    const int i = idx[k];
+    for (int q = 0; q < STAB_MAX_NEV; ++q) {
+      eval[(size_t)i * STAB_MAX_NEV + q] = ev[(size_t)k * STAB_MAX_NEV + q];
+      if (resid) resid[(size_t)i * STAB_MAX_NEV + q] = rs[(size_t)k * STAB_MAX_NEV + q];
+    }
+    info[i] = inf[k];
+    const size_t len = off[i + 1] - off[i];
+    if (evec) memcpy(evec + off[i], vec.data() + so, sizeof(stab_z) * len);
+    so += len;
+  }
+  if (stats) {
+    stats->ms_eig += fst.ms_total;
+    stats->ms_total += fst.ms_total;
+    stats->nconv += fst.nconv;
+    stats->ms_arnoldi += fst.ms_arnoldi;
+    stats->op_applies += fst.op_applies;
+    stats->solve_bytes += fst.solve_bytes;
   }
   return STAB_OK;
 }

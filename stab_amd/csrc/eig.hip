@@ -26,56 +26,104 @@
 struct KSState {
   int j0;        // first column to expand
   int k;         // kept columns after the last restart
-  int done;      // 1 when every wanted pair has converged
+  int done;      // 1: every wanted pair converged; 2: stopped on stagnation
   int cycles;
   double resmax; // max relative Ritz residual estimate of the wanted pairs
+  double best;   // smallest resmax seen
+  int stall;     // cycles without a 10x improvement of best
+  int irbad;     // 1 if a refined solve contracted poorly (FP32 factors inadequate)
+  int steps;     // operator applications so far
+  int pad;
 };
 
 // ---------------------------------------------------------------------------
-// y <- M^-1 y with the blocked factors of k_lu (scales applied by the caller).
-// y, tmp in LDS (tmp: NB entries).  All threads call.
+// y <- M^-1 y with the factors of k_lu and the 64 x 64 diagonal-block
+// inverses of k_blkinv (scales applied by the caller).  Per block of 64 rows:
+// y_top <- inv(D) y_top as a GEMV (4 threads per row), then the rows below
+// (forward) or above (backward) are updated with independent accumulators
+// for all rows a thread owns.  y, work in LDS (npad), tmp in LDS (>= 64).
 // ---------------------------------------------------------------------------
-template <typename T, int NB, int NT>
+#define SOLVE_BS 64
+template <typename T, int NT>
 __device__ void lu_solve(const PointDev &P, const cx<T> *__restrict__ A, const int *__restrict__ ipiv,
-                         const cx<T> *__restrict__ Lb, const cx<T> *__restrict__ Ub, zd *y, zd *tmp) {
+                         const cx<T> *__restrict__ blk, zd *y, zd *tmp, zd *work) {
+  constexpr int BS = SOLVE_BS, RS = 6;      // npad <= RS * NT
   const int n = P.npad, lda = P.npad, t = threadIdx.x;
-  for (int k0 = 0; k0 < n; k0 += NB) {
-    if (t == 0) {
-      for (int c = 0; c < NB; ++c) {
-        const int p = ipiv[k0 + c];
-        if (p != k0 + c && p > k0 + c && p < n) { const zd a = y[k0 + c]; y[k0 + c] = y[p]; y[p] = a; }
+  const int *perm = ipiv + n;
+  const int nblk = (n + BS - 1) / BS;
+  const cx<T> *Li = blk, *Ui = blk + (size_t)nblk * BS * BS;   // layout of k_blkinv
+  // y <- P y (one gather through `work`)
+  for (int r = t; r < n; r += NT) work[r] = y[perm[r]];
+  __syncthreads();
+  for (int r = t; r < n; r += NT) y[r] = work[r];
+  __syncthreads();
+  const int ti = t >> 2, tq = t & 3;         // row of the block GEMV, quarter of the columns
+  for (int k0 = 0; k0 < n; k0 += BS) {
+    const int bs = min(BS, n - k0);
+    if (ti < bs) {
+      zd s = czero<double>();
+      const cx<T> *row = Li + (size_t)(k0 / BS) * BS * BS + ti * BS;
+      for (int k = tq; k <= ti; k += 4) s = cfma(s, cvt<double>(row[k]), y[k0 + k]);
+      s.x += __shfl_xor(s.x, 1); s.y += __shfl_xor(s.y, 1);
+      s.x += __shfl_xor(s.x, 2); s.y += __shfl_xor(s.y, 2);
+      if (tq == 0) tmp[ti] = s;
+    }
+    __syncthreads();
+    zd acc[RS];
+#pragma unroll
+    for (int j = 0; j < RS; ++j) {
+      const int r = k0 + bs + t + j * NT;
+      acc[j] = (r < n) ? y[r] : czero<double>();
+    }
+    for (int c = 0; c < bs; ++c) {
+      const zd tc = tmp[c];
+      const cx<T> *col = A + (size_t)(k0 + c) * lda;
+#pragma unroll
+      for (int j = 0; j < RS; ++j) {
+        const int r = k0 + bs + t + j * NT;
+        if (r < n) acc[j] = cfms(acc[j], cvt<double>(col[r]), tc);
       }
     }
-    __syncthreads();
-    if (t < NB) {
-      zd s = czero<double>();
-      for (int k = 0; k <= t; ++k) s = cfma(s, cvt<double>(Lb[(size_t)k0 * NB + t * NB + k]), y[k0 + k]);
-      tmp[t] = s;
+#pragma unroll
+    for (int j = 0; j < RS; ++j) {
+      const int r = k0 + bs + t + j * NT;
+      if (r < n) y[r] = acc[j];
     }
-    __syncthreads();
-    if (t < NB) y[k0 + t] = tmp[t];
-    for (int r = k0 + NB + t; r < n; r += NT) {
-      zd acc = y[r];
-#pragma unroll 8
-      for (int c = 0; c < NB; ++c) acc = cfms(acc, cvt<double>(A[(size_t)(k0 + c) * lda + r]), tmp[c]);
-      y[r] = acc;
-    }
+    if (t < bs) y[k0 + t] = tmp[t];
     __syncthreads();
   }
-  for (int k0 = n - NB; k0 >= 0; k0 -= NB) {
-    if (t < NB) {
+  for (int b = nblk - 1; b >= 0; --b) {
+    const int k0 = b * BS, bs = min(BS, n - k0);
+    if (ti < bs) {
       zd s = czero<double>();
-      for (int k = t; k < NB; ++k) s = cfma(s, cvt<double>(Ub[(size_t)k0 * NB + t * NB + k]), y[k0 + k]);
-      tmp[t] = s;
+      const cx<T> *row = Ui + (size_t)b * BS * BS + ti * BS;
+      for (int k = ti + ((tq - ti) & 3); k < bs; k += 4) s = cfma(s, cvt<double>(row[k]), y[k0 + k]);
+      s.x += __shfl_xor(s.x, 1); s.y += __shfl_xor(s.y, 1);
+      s.x += __shfl_xor(s.x, 2); s.y += __shfl_xor(s.y, 2);
+      if (tq == 0) tmp[ti] = s;
     }
     __syncthreads();
-    if (t < NB) y[k0 + t] = tmp[t];
-    for (int r = t; r < k0; r += NT) {
-      zd acc = y[r];
-#pragma unroll 8
-      for (int c = 0; c < NB; ++c) acc = cfms(acc, cvt<double>(A[(size_t)(k0 + c) * lda + r]), tmp[c]);
-      y[r] = acc;
+    zd acc[RS];
+#pragma unroll
+    for (int j = 0; j < RS; ++j) {
+      const int r = t + j * NT;
+      acc[j] = (r < k0) ? y[r] : czero<double>();
     }
+    for (int c = 0; c < bs; ++c) {
+      const zd tc = tmp[c];
+      const cx<T> *col = A + (size_t)(k0 + c) * lda;
+#pragma unroll
+      for (int j = 0; j < RS; ++j) {
+        const int r = t + j * NT;
+        if (r < k0) acc[j] = cfms(acc[j], cvt<double>(col[r]), tc);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < RS; ++j) {
+      const int r = t + j * NT;
+      if (r < k0) y[r] = acc[j];
+    }
+    if (t < bs) y[k0 + t] = tmp[t];
     __syncthreads();
   }
 }
@@ -114,22 +162,33 @@ __device__ double block_norm2(const zd *x, int len, zd *red) {
 }
 
 // Shared-memory layout of the operator kernels (dynamic LDS, zd units).
+// The full layout serves refinement and the spatial operator; the compact one
+// (plain FP64, temporal only) keeps just the solve vector and the Krylov
+// vector, so more work-groups fit on a CU.
 struct OpLds {
-  zd *y, *rhs, *res, *X1, *X2, *z, *red, *tmp;
-  __device__ OpLds(zd *base, const PointDev &P, int npadmax, int nymax) {
-    y = base;
-    rhs = y + npadmax;
-    res = rhs + npadmax;
-    X1 = res + npadmax;
-    X2 = X1 + 5 * nymax;
-    z = X2 + 5 * nymax;
-    red = z + 2 * npadmax;
+  zd *y, *rhs, *res, *X1, *X2, *z, *red, *tmp, *w2;
+  // nvecmax: longest Krylov vector (2 n for spatial points).  w2, the gather
+  // buffer of the solves, aliases X1/X2 (10 ny >= npad), which are idle then.
+  __device__ OpLds(zd *base, const PointDev &P, int npadmax, int nymax, int full, int nvecmax) {
+    red = base;
     tmp = red + 64;
+    y = tmp + 64;
+    z = y + npadmax;
+    if (full) {
+      rhs = z + nvecmax;
+      res = rhs + npadmax;
+      X1 = res + npadmax;
+      X2 = X1 + 5 * nymax;
+      w2 = X1;
+    } else {
+      rhs = res = X1 = X2 = w2 = nullptr;
+    }
   }
 };
 
-__host__ __device__ inline size_t op_lds_bytes(int npadmax, int nymax) {
-  return sizeof(zd) * ((size_t)3 * npadmax + 10 * nymax + 2 * npadmax + 64 + 64);
+__host__ __device__ inline size_t op_lds_bytes(int npadmax, int nymax, int full, int nvecmax) {
+  return sizeof(zd) * (128 + (size_t)npadmax + (size_t)nvecmax +
+                       (full ? 2 * (size_t)npadmax + 10 * (size_t)nymax : 0));
 }
 
 // y = M^-1 rhs in FP64 quality: scaled solve with the factors, then
@@ -139,23 +198,24 @@ __host__ __device__ inline size_t op_lds_bytes(int npadmax, int nymax) {
 // (the FP64 rounding floor of the residual has been reached).  The contraction
 // per step is about kappa(M) u32.
 template <typename T, int NB, int NT>
-__device__ void shifted_solve(const PointDev &P, const cx<T> *A, const int *ipiv, const cx<T> *Lb, const cx<T> *Ub,
+__device__ bool shifted_solve(const PointDev &P, const cx<T> *A, const int *ipiv, const cx<T> *Lb, const cx<T> *Ub,
                               const double *dr, const double *dc, const zd *coef, const double *ops, int ir,
                               OpLds &L) {
   const int n = P.npad, t = threadIdx.x;
   for (int r = t; r < n; r += NT) L.y[r] = (r < P.n) ? L.rhs[r] * dr[r] : czero<double>();
   __syncthreads();
-  lu_solve<T, NB, NT>(P, A, ipiv, Lb, Ub, L.y, L.tmp);
+  lu_solve<T, NT>(P, A, ipiv, Lb, L.y, L.tmp, L.w2);
   for (int r = t; r < n; r += NT) L.y[r] = L.y[r] * dc[r];
   __syncthreads();
   const bool adaptive = ir < 0;
   const int steps = adaptive ? -ir : ir;
   double dprev = 0.0;
+  bool ok = !adaptive;
   for (int s = 0; s < steps; ++s) {
     pencil_apply<NT>(P, coef, ops, FORM_SHIFTED, P.shift, L.y, L.res, L.X1, L.X2, L.red);
     for (int r = t; r < n; r += NT) L.res[r] = (r < P.n) ? (L.rhs[r] - L.res[r]) * dr[r] : czero<double>();
     __syncthreads();
-    lu_solve<T, NB, NT>(P, A, ipiv, Lb, Ub, L.res, L.tmp);
+    lu_solve<T, NT>(P, A, ipiv, Lb, L.res, L.tmp, L.w2);
     zd nrm[2] = {czero<double>(), czero<double>()};
     for (int r = t; r < n; r += NT) {
       const zd d = L.res[r] * dc[r];
@@ -165,25 +225,44 @@ __device__ void shifted_solve(const PointDev &P, const cx<T> *A, const int *ipiv
     }
     if (adaptive) {
       block_reduce<NT, 2>(nrm, L.red);
+      // the FP32 factors are adequate for this shift when the first correction
+      // is below 1e-2 of the solution (refinement then contracts fast)
+      if (s == 0) ok = nrm[0].x <= 1e-4 * nrm[1].x;
       if (nrm[0].x <= 1e-28 * nrm[1].x || (s > 0 && nrm[0].x > 0.25 * dprev)) break;
       dprev = nrm[0].x;
     } else {
       __syncthreads();
     }
   }
+  return ok;
 }
 
 // w = T v for point P; v read from global (length nvec), w left in L.z.
-template <typename T, int NB, int NT>
-__device__ void apply_op(const PointDev &P, const cx<T> *A, const int *ipiv, const cx<T> *Lb, const cx<T> *Ub,
+// FULL = false compiles only the plain FP64 temporal operator.
+template <typename T, int NB, int NT, bool FULL = true>
+__device__ bool apply_op(const PointDev &P, const cx<T> *A, const int *ipiv, const cx<T> *Lb, const cx<T> *Ub,
                          const double *dr, const double *dc, const zd *coef, const double *ops, int ir,
                          const zd *v, OpLds &L) {
   const int n = P.n, t = threadIdx.x;
+  if (!FULL || (P.itype == 1 && L.rhs == nullptr)) {
+    // plain FP64 temporal: y = B v in place, one solve, no scaling
+    for (int r = t; r < n; r += NT) L.z[r] = v[r];
+    __syncthreads();
+    pencil_apply<NT>(P, coef, ops, FORM_BM, P.shift, L.z, L.y, L.X1, L.X2, L.red);
+    for (int r = n + t; r < P.npad; r += NT) L.y[r] = czero<double>();
+    __syncthreads();
+    lu_solve<T, NT>(P, A, ipiv, Lb, L.y, L.tmp, L.z);
+    for (int r = t; r < n; r += NT) L.z[r] = L.y[r];
+    __syncthreads();
+    return true;
+  }
+  if constexpr (!FULL) return true;
+  bool ok;
   if (P.itype == 1) {
     for (int r = t; r < n; r += NT) L.z[r] = v[r];
     __syncthreads();
     pencil_apply<NT>(P, coef, ops, FORM_BM, P.shift, L.z, L.rhs, L.X1, L.X2, L.red);
-    shifted_solve<T, NB, NT>(P, A, ipiv, Lb, Ub, dr, dc, coef, ops, ir, L);
+    ok = shifted_solve<T, NB, NT>(P, A, ipiv, Lb, Ub, dr, dc, coef, ops, ir, L);
     for (int r = t; r < n; r += NT) L.z[r] = L.y[r];
     __syncthreads();
   } else {
@@ -195,10 +274,11 @@ __device__ void apply_op(const PointDev &P, const cx<T> *A, const int *ipiv, con
     pencil_apply<NT>(P, coef, ops, FORM_C2, sg, L.z + n, L.res, L.X1, L.X2, L.red);
     for (int r = t; r < n; r += NT) L.rhs[r] = czero<double>() - L.rhs[r] - L.res[r];
     __syncthreads();
-    shifted_solve<T, NB, NT>(P, A, ipiv, Lb, Ub, dr, dc, coef, ops, ir, L);
+    ok = shifted_solve<T, NB, NT>(P, A, ipiv, Lb, Ub, dr, dc, coef, ops, ir, L);
     for (int r = t; r < n; r += NT) { L.z[n + r] = L.z[r] + sg * L.y[r]; L.z[r] = L.y[r]; }
     __syncthreads();
   }
+  return ok;
 }
 
 // Deterministic start vector: T applied to a hashed pseudo-random vector.
@@ -211,18 +291,18 @@ __device__ inline double hash01(unsigned a, unsigned b) {
 // ---------------------------------------------------------------------------
 // Arnoldi expansion from column j0 to m (CGS2).  V column-major, ld = ldv.
 // ---------------------------------------------------------------------------
-template <typename T, int NB, int NT>
-__global__ __launch_bounds__(NT) void k_arnoldi(const PointDev *__restrict__ pts, const cx<T> *__restrict__ mats,
+template <typename T, int NB, int NT, bool FULL>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FULL ? 2 : 4))) void k_arnoldi(const PointDev *__restrict__ pts, const cx<T> *__restrict__ mats,
                                                 const int *__restrict__ pivs, const cx<T> *__restrict__ blks,
                                                 const double *__restrict__ scales, const zd *__restrict__ coef,
                                                 const double *__restrict__ ops, zd *__restrict__ Vall,
                                                 zd *__restrict__ small, KSState *__restrict__ st, int m, int ir,
-                                                int npadmax, int nymax, int start) {
+                                                int npadmax, int nymax, int nvecmax, int start) {
   extern __shared__ zd lds[];
   const PointDev &P = pts[blockIdx.x];
-  KSState S = st[blockIdx.x];
-  if (S.done) return;
-  OpLds L(lds, P, npadmax, nymax);
+  if (st[blockIdx.x].done) return;
+  int j0 = st[blockIdx.x].j0;
+  OpLds L(lds, P, npadmax, nymax, FULL ? 1 : 0, nvecmax);
   const int t = threadIdx.x, nv = P.nvec;
   const size_t ldv = (size_t)nv;
   zd *V = Vall + P.vec_off;
@@ -230,47 +310,48 @@ __global__ __launch_bounds__(NT) void k_arnoldi(const PointDev *__restrict__ pts
   const int ldh = KS_MMAX + 1;
   const cx<T> *A = mats + P.mat_off;
   const int *ipiv = pivs + P.piv_off;
-  const cx<T> *Lb = blks + P.blk_off, *Ub = Lb + (size_t)P.npad * NB;
+  const cx<T> *Lb = blks + P.blk_off, *Ub = nullptr;
   const double *dr = scales + P.scale_off, *dc = dr + P.npad;
 
+  bool allok = true;
   if (start) {
     // v0 = T r / ||T r||, r pseudo-random
     for (int r = t; r < nv; r += NT) V[r] = mk(hash01(blockIdx.x, 2 * r), hash01(blockIdx.x, 2 * r + 1));
     __threadfence_block();
     __syncthreads();
-    apply_op<T, NB, NT>(P, A, ipiv, Lb, Ub, dr, dc, coef, ops, ir, V, L);
+    allok &= apply_op<T, NB, NT, FULL>(P, A, ipiv, Lb, Ub, dr, dc, coef, ops, ir, V, L);
     const double nrm = sqrt(block_norm2<NT>(L.z, nv, L.red));
     for (int r = t; r < nv; r += NT) V[r] = L.z[r] * (1.0 / nrm);
     for (int e = t; e < ldh * KS_MMAX; e += NT) H[e] = czero<double>();
     __syncthreads();
-    S.j0 = 0; S.k = 0;
+    j0 = 0;
   }
-  for (int j = S.j0; j < m; ++j) {
-    apply_op<T, NB, NT>(P, A, ipiv, Lb, Ub, dr, dc, coef, ops, ir, V + (size_t)j * ldv, L);
+  for (int j = j0; j < m; ++j) {
+    allok &= apply_op<T, NB, NT, FULL>(P, A, ipiv, Lb, Ub, dr, dc, coef, ops, ir, V + (size_t)j * ldv, L);
     // two passes of classical Gram-Schmidt against V[:, 0..j]
     for (int pass = 0; pass < 2; ++pass) {
-      for (int l0 = 0; l0 <= j; l0 += 8) {
-        zd h[8];
+      for (int l0 = 0; l0 <= j; l0 += 4) {
+        zd h[4];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) h[k] = czero<double>();
+        for (int k = 0; k < 4; ++k) h[k] = czero<double>();
         for (int r = t; r < nv; r += NT) {
           const zd w = L.z[r];
 #pragma unroll
-          for (int k = 0; k < 8; ++k)
+          for (int k = 0; k < 4; ++k)
             if (l0 + k <= j) h[k] = cfmac(h[k], V[(size_t)(l0 + k) * ldv + r], w);
         }
-        block_reduce<NT, 8>(h, L.red);
+        block_reduce<NT, 4>(h, L.red);
         for (int r = t; r < nv; r += NT) {
           zd w = L.z[r];
 #pragma unroll
-          for (int k = 0; k < 8; ++k)
+          for (int k = 0; k < 4; ++k)
             if (l0 + k <= j) w = cfms(w, V[(size_t)(l0 + k) * ldv + r], h[k]);
           L.z[r] = w;
         }
-        if (t < 8 && l0 + t <= j) {
+        if (t < 4 && l0 + t <= j) {
           zd hv = h[0];
 #pragma unroll
-          for (int k = 0; k < 8; ++k) if (k == t) hv = h[k];
+          for (int k = 0; k < 4; ++k) if (k == t) hv = h[k];
           H[(size_t)j * ldh + l0 + t] = H[(size_t)j * ldh + l0 + t] + hv;
         }
         __syncthreads();
@@ -282,7 +363,12 @@ __global__ __launch_bounds__(NT) void k_arnoldi(const PointDev *__restrict__ pts
     if (t == 0) H[(size_t)j * ldh + j + 1] = mk(nrm, 0.0);
     __syncthreads();
   }
-  if (t == 0) { S.j0 = m; st[blockIdx.x] = S; }
+  if (t == 0) {
+    st[blockIdx.x].steps += (m - j0) + (start ? 1 : 0);
+    st[blockIdx.x].j0 = m;
+    if (start) st[blockIdx.x].k = 0;
+    if (!allok) st[blockIdx.x].irbad = 1;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -520,10 +606,14 @@ __global__ __launch_bounds__(64) void k_schur(const PointDev *__restrict__ pts, 
     double mx = 0.0;
     for (int i = 0; i < nev; ++i) mx = fmax(mx, s_est[i]);
     S.resmax = mx;
-    S.done = (mx <= tol) ? 1 : 0;
     S.cycles += 1;
+    if (S.cycles == 1 || mx < 0.1 * S.best) { S.best = fmin(mx, S.cycles == 1 ? mx : S.best); S.stall = 0; }
+    else S.stall += 1;
+    S.best = fmin(S.best, mx);
+    // converged, or no tenfold progress in three cycles (an ill-conditioned
+    // cluster among the wanted pairs): stop this point
+    S.done = (mx <= tol) ? 1 : ((S.stall >= 3) ? 2 : 0);
     st[blockIdx.x] = S;
-    // keep hlast for the restart in H(m, m-1) (column m-1, row m) untouched
   }
 }
 
@@ -584,10 +674,10 @@ __global__ __launch_bounds__(NT) void k_final(const PointDev *__restrict__ pts, 
                                               const zd *__restrict__ small, const zd *__restrict__ coef,
                                               const double *__restrict__ ops, const double *__restrict__ norms,
                                               int m, zd *__restrict__ eval, zd *__restrict__ evec,
-                                              double *__restrict__ resid, int npadmax, int nymax) {
+                                              double *__restrict__ resid, int npadmax, int nymax, int nvecmax) {
   extern __shared__ zd lds[];
   const PointDev &P = pts[blockIdx.x];
-  OpLds L(lds, P, npadmax, nymax);
+  OpLds L(lds, P, npadmax, nymax, 1, nvecmax);
   const int t = threadIdx.x, nv = P.nvec, n = P.n;
   const size_t ldv = (size_t)nv;
   const zd *V = Vall + P.vec_off;
@@ -710,10 +800,10 @@ __global__ __launch_bounds__(NT) void k_solve_one(const PointDev *__restrict__ p
                                                   zd *__restrict__ x, int ir, int npadmax, int nymax) {
   extern __shared__ zd lds[];
   const PointDev &P = pts[blockIdx.x];
-  OpLds L(lds, P, npadmax, nymax);
+  OpLds L(lds, P, npadmax, nymax, 1, npadmax);
   const cx<T> *A = mats + P.mat_off;
   const int *ipiv = pivs + P.piv_off;
-  const cx<T> *Lb = blks + P.blk_off, *Ub = Lb + (size_t)P.npad * NB;
+  const cx<T> *Lb = blks + P.blk_off, *Ub = nullptr;
   const double *dr = scales + P.scale_off, *dc = dr + P.npad;
   for (int r = threadIdx.x; r < P.n; r += NT) L.rhs[r] = b[r];
   __syncthreads();

@@ -209,6 +209,19 @@ class Solver:
                                                 x.ctypes.data_as(zp), C.byref(info)))
         return x, info.value
 
+    def lu_factors(self, pt: Point, mode: int = _lib.MODE_PLAIN_F64):
+        """Packed LU factors, row permutation and equilibration scales of the
+        shifted matrix (see stab_lu_factors)."""
+        npad = -(-5 * pt.ny // 32) * 32
+        lu = np.zeros((npad, npad), dtype=np.complex128, order="F")
+        perm = np.zeros(npad, dtype=np.int32)
+        scales = np.zeros(2 * npad)
+        info = C.c_int32()
+        self._check(self.lib.stab_lu_factors(self.ctx, C.byref(pt), mode, lu.ctypes.data_as(C.POINTER(Z)),
+                                             perm.ctypes.data_as(C.POINTER(C.c_int32)),
+                                             scales.ctypes.data_as(C.POINTER(C.c_double)), C.byref(info)))
+        return lu, perm, scales[:npad], scales[npad:], info.value
+
     def solve(self, points: Sequence[Point], mode: int = _lib.MODE_PLAIN_F64, want_vectors: bool = True):
         npts = len(points)
         arr = (Point * npts)(*points)
