@@ -22,6 +22,36 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
+FORTRAN_DIR = os.path.join(HERE, "fortran")
+FORTRAN_BIN = os.path.join(HERE, "stab_gpu")
+
+
+def fortran_compiler():
+    for cand in (shutil.which("amdflang"), "/opt/rocm/bin/amdflang", shutil.which("flang"), shutil.which("gfortran")):
+        if cand and os.path.exists(cand):
+            return cand
+    return None
+
+
+def build_fortran() -> str | None:
+    """Compile the Fortran host driver (ISO_C_BINDING onto the C ABI) next to
+    the library; returns its path, or None when no Fortran compiler exists."""
+    fc = fortran_compiler()
+    if fc is None:
+        return None
+    work = os.path.join(ROOT, "_build", "fortran")
+    os.makedirs(work, exist_ok=True)
+    tmp = FORTRAN_BIN + ".tmp"
+    cmd = [fc, "-O2", "-J", work, os.path.join(FORTRAN_DIR, "stab_hip_iface.f90"),
+           os.path.join(FORTRAN_DIR, "stab_gpu.f90"), "-L" + HERE, "-lstab_hip",
+           "-Wl,-rpath,$ORIGIN", "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True, cwd=work)
+    if r.returncode != 0:
+        raise RuntimeError("Fortran build failed:\n" + r.stdout + r.stderr)
+    os.replace(tmp, FORTRAN_BIN)
+    return FORTRAN_BIN
+
+
 def build(verbose: bool = False) -> str:
     """Compile the HIP library (always recompiles) and return its path."""
     tmp = OUT + ".tmp"
@@ -32,6 +62,7 @@ def build(verbose: bool = False) -> str:
     if verbose:
         print(r.stderr)
     os.replace(tmp, OUT)
+    build_fortran()
     return OUT
 
 

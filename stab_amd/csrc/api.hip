@@ -150,7 +150,7 @@ int build_batch(stab_ctx *c, int npts, const stab_point *pts, size_t elem, int m
     if (!B.op_off.count(pts[i].ny)) {
       B.op_off[pts[i].ny] = (int)ooff;
       B.nys.push_back(pts[i].ny);
-      ooff += 2LL * pts[i].ny * pts[i].ny + pts[i].ny;
+      ooff += 2LL * pts[i].ny * pts[i].ny + 4LL * pts[i].ny + 1;   // D1, D2, Dt2 wall row, row sums
     }
   }
   long long coff = 0, moff = 0, voff = 0, pvoff = 0, boff = 0, soff = 0, eoff = 0;
@@ -241,11 +241,12 @@ int run_prep(stab_ctx *c, int npts, Batch &B, bool equilibrate) {
   hipLaunchKernelGGL(k_nodes, dim3(npts), dim3(256), 0, c->stream, B.pts, B.ops, B.prof, B.coef, B.grid);
   hipLaunchKernelGGL(k_fill, dim3(64), dim3(256), 0, c->stream, B.norms, (size_t)4 * npts, 0.0);
   hipLaunchKernelGGL(k_fill, dim3(256), dim3(256), 0, c->stream, B.scales, B.nscales, 1.0);
-  const int rb = (B.npadmax + 255) / 256;
-  hipLaunchKernelGGL(k_rowstats, dim3(rb, npts), dim3(256), 0, c->stream, B.pts, B.coef, B.ops, B.norms, B.scales,
-                     equilibrate ? 1 : 0);
-  if (equilibrate)
+  hipLaunchKernelGGL(k_norms, dim3(npts), dim3(256), 0, c->stream, B.pts, B.coef, B.ops, B.norms);
+  if (equilibrate) {
+    const int rb = (B.npadmax + 255) / 256;
+    hipLaunchKernelGGL(k_rowstats, dim3(rb, npts), dim3(256), 0, c->stream, B.pts, B.coef, B.ops, B.scales);
     hipLaunchKernelGGL(k_colstats, dim3(rb, npts), dim3(256), 0, c->stream, B.pts, B.coef, B.ops, B.scales);
+  }
   HIPCHECK(c, hipGetLastError());
   return STAB_OK;
 }
@@ -417,7 +418,7 @@ int stab_grid(stab_ctx *c, const stab_point *pt, double *y, double *eta, double 
   std::vector<double> g(9 * 256);
   HIPCHECK(c, hipMemcpyAsync(g.data(), B.grid, sizeof(double) * g.size(), hipMemcpyDeviceToHost, c->stream));
   const int ny = pt->ny;
-  std::vector<double> ops(2 * ny * ny + ny);
+  std::vector<double> ops(2 * ny * ny);
   HIPCHECK(c, hipMemcpyAsync(ops.data(), B.ops, sizeof(double) * ops.size(), hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   if (y) memcpy(y, g.data(), sizeof(double) * ny);

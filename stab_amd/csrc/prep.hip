@@ -70,6 +70,24 @@ __global__ void k_cheb(int ny, double *__restrict__ D1, double *__restrict__ D2,
     for (int k = 0; k < ny - 1; ++k) s += D1[(size_t)k * ny + (ny - 1)] * D1[(size_t)j * ny + k];
     dt2w[j] = s;
   }
+  __syncthreads();
+  // row sums for the closed-form Frobenius norms of the pencil (k_norms):
+  // S11_i = sum_j D1_ij^2, S22_i = sum_j D2_ij^2, S12_i = sum_j D1_ij D2_ij,
+  // and Sdt = sum_j Dt2(wall, j)^2
+  double *sums = dt2w + ny;
+  for (int i = threadIdx.x; i < ny; i += blockDim.x) {
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (int j = 0; j < ny; ++j) {
+      const double d1 = D1[(size_t)j * ny + i], d2 = D2[(size_t)j * ny + i];
+      a += d1 * d1; b += d2 * d2; c += d1 * d2;
+    }
+    sums[i] = a; sums[ny + i] = b; sums[2 * ny + i] = c;
+  }
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int j = 0; j < ny; ++j) s += dt2w[j] * dt2w[j];
+    sums[3 * ny] = s;
+  }
 }
 
 // ---------------------------------------------------------------------------
