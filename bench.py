@@ -58,6 +58,21 @@ def make_points(al, be, re, ny, nev=8):
     return pts
 
 
+def load_traffic(kernel: str, key: str):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    whose workload-key line equals `key` (profiles/*_hbm_traffic.md), or None."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*_hbm_traffic.md"))):
+        text = open(path).read()
+        if f"workload-key: {key}\n" not in text:
+            continue
+        for line in text.splitlines():
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cells) == 5 and cells[0].startswith(kernel + "<") or (len(cells) == 5 and cells[0] == kernel):
+                return float(cells[4]) * 1e9
+    return None
+
+
 def cpu_worker(args):
     """One point of the CPU baseline: the oracle's reference method
     (ZGESV + ZGEEV) timed, plus ZGGEV on the same pencil for the check."""
@@ -109,6 +124,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="run one device-to-device copy of MIB MiB first (reference kernel for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
@@ -122,6 +139,15 @@ def main():
     pts = make_points(al, be, re, ny, args.nev)
     S = sv.Solver(0)
     S.set_profile(0, tab[:, 0], tab[:, 1:6])
+    copy_gbs = None
+    if args.copy_calibration > 0:
+        nbytes = args.copy_calibration << 20
+        ms = S.copy_probe(nbytes, 1)
+        print(f"# copy calibration: k_copy16 {args.copy_calibration} MiB read + {args.copy_calibration} MiB "
+              f"written, {ms:.3f} ms", file=sys.stderr, flush=True)
+    else:
+        ms = S.copy_probe(1 << 30, 10)
+        copy_gbs = 2.0 * (1 << 30) / (ms * 1e-3) / 1e9
 
     def run(mode, steps, warm):
         for _ in range(warm):
@@ -198,8 +224,11 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "k_arnoldi (shift-invert Krylov expansion: blocked triangular "
                                                "solves + CGS2)",
                      "achieved": plain["arnoldi_gbs"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": (plain["arnoldi_gbs"] or 0.0) / PEAK_HBM_GBS, "traffic": None,
-                     "algorithmic_bytes": "16 npad^2 per operator application (one pass over the FP64 LU factors)"},
+                     "frac": (plain["arnoldi_gbs"] or 0.0) / PEAK_HBM_GBS,
+                     "traffic": load_traffic("k_arnoldi", f"temporal-sweep points={args.points} seed={args.seed} "
+                                                          f"nev={args.nev} mode=plain"),
+                     "algorithmic_bytes": "16 npad^2 per operator application (one pass over the FP64 LU factors)",
+                     "measured_copy_gbs": copy_gbs},
         "lu": {"kernel": "k_lu (batched complex LU, FP64)", "achieved_tflops": plain["lu_tflops"],
                "peak_tflops": PEAK_FP64_MFMA_TFLOPS, "frac": plain["lu_tflops"] / PEAK_FP64_MFMA_TFLOPS},
         "cpu_baseline": cpu,

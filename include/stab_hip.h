@@ -171,6 +171,13 @@ int stab_lu_factors(stab_ctx *ctx, const stab_point *pt, int mode, stab_z *lu, i
 /* Size in bytes of the evec buffer stab_solve needs for these points. */
 size_t stab_evec_bytes(int npts, const stab_point *pts);
 
+/* Measurement utility with no counterpart in the reference: copies `bytes`
+ * (a multiple of 16) between two device buffers `reps` times with a plain
+ * 16-byte-per-thread kernel and returns the mean time of one copy.  Gives the
+ * achievable HBM bandwidth (2 bytes moved per byte copied) and the reference
+ * kernel that scales a profiler's byte counters. */
+int stab_copy_probe(stab_ctx *ctx, int64_t bytes, int reps, double *ms_per_copy);
+
 #ifdef __cplusplus
 }
 #endif

@@ -344,6 +344,13 @@ int env_int(const char *name, int dflt) {
 
 }  // namespace
 
+// Plain device copy, one 16-byte load and store per thread: the reference
+// kernel for HBM bandwidth and for scaling the byte counters of a profiler.
+__global__ __launch_bounds__(256) void k_copy16(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n16) {
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n16) dst[i] = src[i];
+}
+
 extern "C" {
 
 int stab_ctx_create(stab_ctx **out, int device) {
@@ -697,6 +704,30 @@ int stab_solve(stab_ctx *c, int npts, const stab_point *pts, int mode, stab_z *e
     stats->op_applies += fst.op_applies;
     stats->solve_bytes += fst.solve_bytes;
   }
+  return STAB_OK;
+}
+
+int stab_copy_probe(stab_ctx *c, int64_t bytes, int reps, double *ms_per_copy) {
+  if (!c) return STAB_E_ARG;
+  if (bytes < 16 || (bytes % 16) != 0 || reps < 1) return fail(c, STAB_E_ARG, "copy probe: bytes must be a positive multiple of 16");
+  HIPCHECK(c, hipSetDevice(c->device));
+  void *a = nullptr, *b = nullptr;
+  int rc;
+  if ((rc = buf_get(c, "copy_src", (size_t)bytes, &a)) != STAB_OK) return rc;
+  if ((rc = buf_get(c, "copy_dst", (size_t)bytes, &b)) != STAB_OK) return rc;
+  const size_t n16 = (size_t)bytes / 16;
+  const dim3 grid((unsigned)((n16 + 255) / 256));
+  // touch both buffers once so the timed copies see resident pages
+  hipLaunchKernelGGL(k_copy16, grid, dim3(256), 0, c->stream, (const uint4 *)b, (uint4 *)a, n16);
+  HIPCHECK(c, hipEventRecord(c->ev[0], c->stream));
+  for (int r = 0; r < reps; ++r)
+    hipLaunchKernelGGL(k_copy16, grid, dim3(256), 0, c->stream, (const uint4 *)a, (uint4 *)b, n16);
+  HIPCHECK(c, hipGetLastError());
+  HIPCHECK(c, hipEventRecord(c->ev[1], c->stream));
+  HIPCHECK(c, hipEventSynchronize(c->ev[1]));
+  float ms = 0.f;
+  HIPCHECK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+  if (ms_per_copy) *ms_per_copy = ms / reps;
   return STAB_OK;
 }
 

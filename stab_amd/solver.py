@@ -222,6 +222,12 @@ class Solver:
                                              scales.ctypes.data_as(C.POINTER(C.c_double)), C.byref(info)))
         return lu, perm, scales[:npad], scales[npad:], info.value
 
+    def copy_probe(self, nbytes: int, reps: int = 10) -> float:
+        """Mean ms of one device-to-device copy of nbytes (stab_copy_probe)."""
+        ms = C.c_double()
+        self._check(self.lib.stab_copy_probe(self.ctx, int(nbytes), int(reps), C.byref(ms)))
+        return ms.value
+
     def solve(self, points: Sequence[Point], mode: int = _lib.MODE_PLAIN_F64, want_vectors: bool = True):
         npts = len(points)
         arr = (Point * npts)(*points)
