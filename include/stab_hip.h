@@ -42,6 +42,20 @@ typedef struct { double re, im; } stab_z;
 #define STAB_INFO_OK         0
 #define STAB_INFO_SINGULAR   1   /* exact zero pivot in the LU of the shifted pencil */
 #define STAB_INFO_NOCONV     2   /* not every wanted pair met the residual tolerance */
+#define STAB_INFO_FALLBACK   3   /* stab_solve_leading: found by the FP64 Krylov-Schur fallback */
+
+/* Leading-mode methods (stab_solve_leading) */
+#define STAB_LEAD_KRYLOV 0   /* Krylov-Schur for pt.nev pairs nearest pt.shift, then the leading one */
+#define STAB_LEAD_INVIT  1   /* residual inverse iteration from pt.shift, FP64 Krylov-Schur fallback */
+
+/* A mode is discrete when its eigenvector, normalised to a largest entry of
+ * modulus 1, stays below STAB_FARFIELD_MAX on the nodes of the outer eighth of
+ * the domain (nodes 1..ceil(ny/8), all five variables), and its phase speed
+ * omega_r / alpha_r is more than STAB_CONTINUUM_BAND away from the edge
+ * velocity 1 (the vorticity/entropy continuous spectrum); a spatial mode must
+ * also travel downstream (alpha_r > 0). */
+#define STAB_FARFIELD_MAX   0.1
+#define STAB_CONTINUUM_BAND 0.01
 
 /* Solve modes */
 #define STAB_MODE_PLAIN_F64  0   /* FP64 LU + FP64 shift-invert Arnoldi */
@@ -150,7 +164,7 @@ int stab_solve(stab_ctx *ctx, int npts, const stab_point *pts, int mode, stab_z 
 /* Factor the shifted matrix of one point (STAB_MAT_SHIFTED) and solve
  * M x = b (n = 5 ny entries).  Replaces the ZGETRF/ZGETRS pair of
  * spatial.f90:978,996 (one right-hand side).  mode STAB_MODE_MIXED factors the
- * row/column-equilibrated matrix in FP32 and applies ir_steps FP64
+ * row-equilibrated matrix in FP32 and applies ir_steps FP64
  * iterative-refinement steps (ir_steps < 0: adaptive, at most -ir_steps, until
  * the correction is below 1e-14 relative); STAB_MODE_PLAIN_F64 factors in FP64 (ir_steps
  * then refine against the same FP64 matrix).  info: STAB_INFO_SINGULAR on an
@@ -163,13 +177,37 @@ int stab_shifted_solve(stab_ctx *ctx, const stab_point *pt, int mode, int ir_ste
  * rows/columns beyond 5 ny hold the identity), unit-lower L below the
  * diagonal and U on and above it; perm[r] is the input row that ends in row r
  * (P A = L U).  STAB_MODE_MIXED factors Dr A Dc in FP32 and returns the row and
- * column scales in scales[0:npad] and scales[npad:2 npad] (scales may be NULL).
+ * column scales in scales[0:npad] and scales[npad:2 npad] (scales may be NULL);
+ * Dr holds powers of two and Dc is the identity.
  * Replaces ZGETRF (spatial.f90:978). */
 int stab_lu_factors(stab_ctx *ctx, const stab_point *pt, int mode, stab_z *lu, int32_t *perm, double *scales,
                     int32_t *info);
 
 /* Size in bytes of the evec buffer stab_solve needs for these points. */
 size_t stab_evec_bytes(int npts, const stab_point *pts);
+
+/* Leading eigenmode per point: the least stable discrete mode (largest
+ * Im omega for temporal points, smallest Im alpha for spatial points).  The
+ * reference leaves this choice to the user, who picks a mode from the sorted
+ * spectrum (temporal.f90:844-879) with getevec -v and a guess
+ * (getevec.f90:160-173, thesis/TStest/run.sh:13-17).
+ *   method STAB_LEAD_KRYLOV: the pt.nev pairs nearest pt.shift (as
+ *     stab_solve), then the leading discrete pair with residual < 1e-10.
+ *   method STAB_LEAD_INVIT: factor at pt.shift (a predicted eigenvalue) in
+ *     `mode` (STAB_MODE_MIXED: equilibrated FP32 LU) and run at most maxit
+ *     residual inverse-iteration corrections x <- x - L(shift)^-1 L(lambda) x
+ *     with FP64 residuals (to a relative residual of 1e-16 or the rounding
+ *     floor).  A point whose residual stays above 1e-13, or that
+ *     lands on a mode that is not discrete, is solved again by
+ *     STAB_LEAD_KRYLOV in FP64 from fallback_shift[p] (pt.shift if NULL) and
+ *     pt.nev, and reports STAB_INFO_FALLBACK.
+ *   eval/resid/farfield/iters/info: npts entries (iters = operator solves);
+ *   evec: NULL or sum over points of nvec (5 ny temporal, 10 ny spatial with
+ *     z = [phi; alpha phi]), normalised as getevec.f90:179-189.
+ * info STAB_INFO_NOCONV: no discrete pair met the bound. */
+int stab_solve_leading(stab_ctx *ctx, int npts, const stab_point *pts, int mode, int method, int maxit,
+                       const stab_z *fallback_shift, stab_z *eval, stab_z *evec, double *resid,
+                       double *farfield, int32_t *iters, int32_t *info, stab_stats *stats);
 
 /* Measurement utility with no counterpart in the reference: copies `bytes`
  * (a multiple of 16) between two device buffers `reps` times with a plain

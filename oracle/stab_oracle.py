@@ -1059,6 +1059,59 @@ def parse_table(text):
 
 
 # ----------------------------------------------------------------------------
+# Leading discrete mode.  The reference leaves the choice of mode to the user
+# (sorted spectrum, temporal.f90:844-879; getevec -v with a guess,
+# getevec.f90:160-173, thesis/TStest/run.sh:13-17).  The rule restated here:
+# a mode is discrete when its normalised eigenvector stays below 0.1 on the
+# nodes of the outer eighth of the domain and its phase speed is more than
+# 0.01 away from the edge velocity (vorticity/entropy continuum); spatial
+# modes must also travel downstream (alpha_r > 0).  The leading mode is the
+# discrete one with the largest Im(omega) (temporal) or smallest Im(alpha)
+# (spatial).
+# ----------------------------------------------------------------------------
+FARFIELD_MAX = 0.1
+CONTINUUM_BAND = 0.01
+
+
+def farfield_ratio(vec, ny):
+    """max |x| over the first ceil(ny/8) nodes (5 variables each) / max |x|."""
+    a = np.abs(np.asarray(vec)[:5 * ny])
+    nf = 5 * ((ny + 7) // 8)
+    m = a.max()
+    return float(a[:nf].max() / m) if m > 0 else 1.0
+
+
+def is_discrete(itype, ev, ff, alpha_r=0.0, omega_r=0.0):
+    if not ff < FARFIELD_MAX:
+        return False
+    if itype == 1:
+        if alpha_r == 0.0:
+            return True
+        c = ev.real / alpha_r
+    else:
+        if not ev.real > 0.0:
+            return False
+        c = omega_r / ev.real
+    return abs(c - 1.0) > CONTINUUM_BAND
+
+
+def leading_mode(itype, w, V, ny, alpha_r=0.0, omega_r=0.0, candidates=None):
+    """Index of the leading discrete mode among `candidates` (default: all
+    finite, nonzero eigenvalues), or -1."""
+    idx = candidates if candidates is not None else [j for j in range(len(w))
+                                                     if np.isfinite(w[j]) and abs(w[j]) > 1e-9]
+    best, key = -1, -np.inf
+    for j in idx:
+        ff = farfield_ratio(V[:, j], ny)
+        if not is_discrete(itype, w[j], ff, alpha_r, omega_r):
+            continue
+        k = w[j].imag if itype == 1 else -w[j].imag
+        if k > key:
+            best, key = j, k
+    return best
+
+
+# ----------------------------------------------------------------------------
 # a14: evec.dat (gfortran sequential unformatted; temporal.f90:883-890)
 # ----------------------------------------------------------------------------
 def read_evec_dat(path):

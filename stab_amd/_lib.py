@@ -16,10 +16,12 @@ STAB_MAX_NEV = 16
 MODE_PLAIN_F64 = 0
 MODE_MIXED = 1
 MAT_A0, MAT_B0, MAT_C0, MAT_C1, MAT_C2, MAT_SHIFTED = range(6)
-INFO_OK, INFO_SINGULAR, INFO_NOCONV = 0, 1, 2
+INFO_OK, INFO_SINGULAR, INFO_NOCONV, INFO_FALLBACK = 0, 1, 2, 3
+LEAD_KRYLOV, LEAD_INVIT = 0, 1
+FARFIELD_MAX, CONTINUUM_BAND = 0.1, 0.01
 
 EXPORTS = ("stab_ctx_create", "stab_ctx_destroy", "stab_last_error", "stab_profile_set", "stab_grid",
-           "stab_pencil", "stab_solve", "stab_evec_bytes", "stab_shifted_solve", "stab_lu_factors", "stab_copy_probe")
+           "stab_pencil", "stab_solve", "stab_evec_bytes", "stab_shifted_solve", "stab_lu_factors", "stab_copy_probe", "stab_solve_leading")
 
 
 class Z(C.Structure):
@@ -68,6 +70,8 @@ def load():
     lib.stab_shifted_solve.argtypes = [vp, C.POINTER(Point), C.c_int, C.c_int, C.POINTER(Z), C.POINTER(Z), ip]
     lib.stab_lu_factors.argtypes = [vp, C.POINTER(Point), C.c_int, C.POINTER(Z), ip, dp, ip]
     lib.stab_copy_probe.argtypes = [vp, C.c_int64, C.c_int, dp]
+    lib.stab_solve_leading.argtypes = [vp, C.c_int, C.POINTER(Point), C.c_int, C.c_int, C.c_int, C.POINTER(Z),
+                                       C.POINTER(Z), C.POINTER(Z), dp, dp, ip, ip, C.POINTER(Stats)]
     lib.stab_evec_bytes.restype = C.c_size_t
     for name in EXPORTS:
         lib[name].restype = lib[name].restype if name in ("stab_last_error", "stab_evec_bytes") else C.c_int

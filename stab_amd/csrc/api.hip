@@ -122,8 +122,11 @@ struct Batch {
   PointDev *pts = nullptr;
   double *ops = nullptr, *prof = nullptr, *grid = nullptr, *scales = nullptr, *norms = nullptr;
   zd *coef = nullptr, *V = nullptr, *small = nullptr, *evec = nullptr, *eval = nullptr;
-  double *resid = nullptr;
+  double *resid = nullptr, *ff = nullptr;
   void *mats = nullptr, *blks = nullptr;
+  int *iters = nullptr, *sel = nullptr, *flag = nullptr;
+  long long *doff = nullptr;
+  zd *lead_vec = nullptr;
   int *pivs = nullptr, *info = nullptr;
   KSState *state = nullptr;
   int npadmax = 0, nymax = 0, nvecmax = 0;
@@ -218,6 +221,16 @@ int build_batch(stab_ctx *c, int npts, const stab_point *pts, size_t elem, int m
   GET("eval", sizeof(zd) * KS_NEVMAX * (size_t)npts, B.eval);
   GET("resid", sizeof(double) * KS_NEVMAX * (size_t)npts, B.resid);
   GET("evec", sizeof(zd) * (eoff > 0 ? eoff : 1), B.evec);
+  GET("ff", sizeof(double) * KS_NEVMAX * (size_t)npts, B.ff);
+  GET("iters", sizeof(int) * npts, B.iters);
+  GET("sel", sizeof(int) * npts, B.sel);
+  GET("flag", sizeof(int) * npts, B.flag);
+  GET("doff", sizeof(long long) * (npts + 1), B.doff);
+  {
+    long long tot = 0;
+    for (int i = 0; i < npts; ++i) tot += B.h[i].nvec;
+    GET("lead_vec", sizeof(zd) * (tot > 0 ? tot : 1), B.lead_vec);
+  }
 #undef GET
   HIPCHECK(c, hipMemcpyAsync(B.pts, B.h.data(), sizeof(PointDev) * npts, hipMemcpyHostToDevice, c->stream));
   for (auto &kv : prof_off) {
@@ -242,11 +255,8 @@ int run_prep(stab_ctx *c, int npts, Batch &B, bool equilibrate) {
   hipLaunchKernelGGL(k_fill, dim3(64), dim3(256), 0, c->stream, B.norms, (size_t)4 * npts, 0.0);
   hipLaunchKernelGGL(k_fill, dim3(256), dim3(256), 0, c->stream, B.scales, B.nscales, 1.0);
   hipLaunchKernelGGL(k_norms, dim3(npts), dim3(256), 0, c->stream, B.pts, B.coef, B.ops, B.norms);
-  if (equilibrate) {
-    const int rb = (B.npadmax + 255) / 256;
-    hipLaunchKernelGGL(k_rowstats, dim3(rb, npts), dim3(256), 0, c->stream, B.pts, B.coef, B.ops, B.scales);
-    hipLaunchKernelGGL(k_colstats, dim3(rb, npts), dim3(256), 0, c->stream, B.pts, B.coef, B.ops, B.scales);
-  }
+  if (equilibrate)   // row scales; the column scales stay 1
+    hipLaunchKernelGGL(k_rowscale, dim3(npts), dim3(256), 0, c->stream, B.pts, B.coef, B.ops, B.scales);
   HIPCHECK(c, hipGetLastError());
   return STAB_OK;
 }
@@ -326,7 +336,7 @@ int run_eig(stab_ctx *c, int npts, Batch &B, const KSParams &kp, int &cycles_max
   nconv = 0;
   for (auto &s : hs) nconv += (s.done == 1);
   hipLaunchKernelGGL(k_final<256>, dim3(npts), dim3(256), lds, c->stream, B.pts, B.V, B.small, B.coef, B.ops, B.norms,
-                     kp.m, B.eval, B.evec, B.resid, B.npadmax, B.nymax, B.nvecmax);
+                     kp.m, B.eval, B.evec, B.resid, B.ff, B.npadmax, B.nymax, B.nvecmax);
   HIPCHECK(c, hipGetLastError());
   return STAB_OK;
 }
@@ -562,10 +572,16 @@ size_t stab_evec_bytes(int npts, const stab_point *pts) {
   return s * sizeof(stab_z);
 }
 
-static int solve_impl(stab_ctx *c, int npts, const stab_point *pts, int mode, stab_z *eval, stab_z *evec,
-                      double *resid, int32_t *info, stab_stats *stats, std::vector<int> *irbad) {
+struct RunInfo {
+  double ms_prep = 0, ms_assemble = 0, ms_lu = 0, ms_eig = 0, ms_total = 0, ms_arnoldi = 0;
+  int cycles = 0, nconv = 0;
+};
+
+// Validate, build the batch, assemble, factor and run Krylov-Schur + k_final;
+// results stay on the device in B.
+static int solve_run(stab_ctx *c, int npts, const stab_point *pts, int mode, Batch &B, RunInfo &ri) {
   if (!c) return STAB_E_ARG;
-  if (npts <= 0 || !pts || !eval || !info) return fail(c, STAB_E_ARG, "npts/pts/eval/info");
+  if (npts <= 0 || !pts) return fail(c, STAB_E_ARG, "npts/pts");
   if (mode != STAB_MODE_PLAIN_F64 && mode != STAB_MODE_MIXED) return fail(c, STAB_E_ARG, "unknown mode %d", mode);
   for (int i = 0; i < npts; ++i) {
     int rc = check_point(c, pts[i], i);
@@ -583,7 +599,6 @@ static int solve_impl(stab_ctx *c, int npts, const stab_point *pts, int mode, st
   if (kp.m > KS_MMAX || kp.keep > 32 || kp.keep < nevmax || kp.keep >= kp.m)
     return fail(c, STAB_E_ARG, "Krylov-Schur sizes m=%d keep=%d nev=%d", kp.m, kp.keep, nevmax);
   const bool f32 = (mode == STAB_MODE_MIXED);
-  Batch B;
   int rc;
   if ((rc = build_batch(c, npts, pts, f32 ? sizeof(zf) : sizeof(zd), kp.m, B)) != STAB_OK) return rc;
   HIPCHECK(c, hipEventRecord(c->ev[0], c->stream));
@@ -600,12 +615,51 @@ static int solve_impl(stab_ctx *c, int npts, const stab_point *pts, int mode, st
   HIPCHECK(c, hipEventRecord(c->ev[2], c->stream));
   if ((rc = f32 ? launch_lu<float, 32>(c, npts, B) : launch_lu<double, 16>(c, npts, B)) != STAB_OK) return rc;
   HIPCHECK(c, hipEventRecord(c->ev[3], c->stream));
-  int cycles = 0, nconv = 0;
-  double ms_arn = 0.0;
-  if ((rc = f32 ? run_eig<float, 32>(c, npts, B, kp, cycles, nconv, ms_arn)
-                : run_eig<double, 16>(c, npts, B, kp, cycles, nconv, ms_arn)) != STAB_OK)
+  if ((rc = f32 ? run_eig<float, 32>(c, npts, B, kp, ri.cycles, ri.nconv, ri.ms_arnoldi)
+                : run_eig<double, 16>(c, npts, B, kp, ri.cycles, ri.nconv, ri.ms_arnoldi)) != STAB_OK)
     return rc;
   HIPCHECK(c, hipEventRecord(c->ev[4], c->stream));
+  HIPCHECK(c, hipEventSynchronize(c->ev[4]));
+  ri.ms_prep = elapsed(c->ev[0], c->ev[1]);
+  ri.ms_assemble = elapsed(c->ev[1], c->ev[2]);
+  ri.ms_lu = elapsed(c->ev[2], c->ev[3]);
+  ri.ms_eig = elapsed(c->ev[3], c->ev[4]);
+  ri.ms_total = elapsed(c->ev[0], c->ev[4]);
+  return STAB_OK;
+}
+
+static void fill_stats(stab_stats *stats, int npts, const stab_point *pts, const Batch &B, const RunInfo &ri,
+                       bool f32, const std::vector<int> &steps) {
+  memset(stats, 0, sizeof *stats);
+  stats->ms_prep = ri.ms_prep;
+  stats->ms_assemble = ri.ms_assemble;
+  stats->ms_lu = ri.ms_lu;
+  stats->ms_eig = ri.ms_eig;
+  stats->ms_total = ri.ms_total;
+  for (int i = 0; i < npts; ++i) {
+    const double n = 5.0 * pts[i].ny;
+    stats->lu_flops += (int64_t)(8.0 / 3.0 * n * n * n);
+    stats->assemble_bytes += (int64_t)((f32 ? 8.0 : 16.0) * B.h[i].npad * B.h[i].npad);
+  }
+  stats->npts = npts;
+  stats->nconv = ri.nconv;
+  stats->restarts_max = ri.cycles;
+  stats->ms_arnoldi = ri.ms_arnoldi;
+  for (int i = 0; i < npts; ++i) {
+    stats->op_applies += steps[i];
+    // factor bytes read per application: one pass over L and U
+    stats->solve_bytes += (double)steps[i] * (f32 ? 8.0 : 16.0) * B.h[i].npad * B.h[i].npad;
+  }
+}
+
+static int solve_impl(stab_ctx *c, int npts, const stab_point *pts, int mode, stab_z *eval, stab_z *evec,
+                      double *resid, int32_t *info, stab_stats *stats, std::vector<int> *irbad) {
+  if (!c) return STAB_E_ARG;
+  if (npts <= 0 || !pts || !eval || !info) return fail(c, STAB_E_ARG, "npts/pts/eval/info");
+  Batch B;
+  RunInfo ri;
+  int rc;
+  if ((rc = solve_run(c, npts, pts, mode, B, ri)) != STAB_OK) return rc;
   std::vector<zd> ev((size_t)KS_NEVMAX * npts);
   std::vector<double> rs((size_t)KS_NEVMAX * npts);
   std::vector<int> inf(npts);
@@ -620,6 +674,7 @@ static int solve_impl(stab_ctx *c, int npts, const stab_point *pts, int mode, st
     HIPCHECK(c, hipMemcpyAsync(evec, B.evec, sizeof(zd) * tot, hipMemcpyDeviceToHost, c->stream));
   }
   HIPCHECK(c, hipStreamSynchronize(c->stream));
+  std::vector<int> steps(npts);
   for (int i = 0; i < npts; ++i) {
     for (int k = 0; k < STAB_MAX_NEV; ++k) {
       const size_t e = (size_t)i * STAB_MAX_NEV + k;
@@ -630,29 +685,9 @@ static int solve_impl(stab_ctx *c, int npts, const stab_point *pts, int mode, st
     }
     info[i] = inf[i] ? STAB_INFO_SINGULAR : (hs[i].done == 1 ? STAB_INFO_OK : STAB_INFO_NOCONV);
     if (irbad) (*irbad)[i] = hs[i].irbad || inf[i];
+    steps[i] = hs[i].steps;
   }
-  if (stats) {
-    memset(stats, 0, sizeof *stats);
-    stats->ms_prep = elapsed(c->ev[0], c->ev[1]);
-    stats->ms_assemble = elapsed(c->ev[1], c->ev[2]);
-    stats->ms_lu = elapsed(c->ev[2], c->ev[3]);
-    stats->ms_eig = elapsed(c->ev[3], c->ev[4]);
-    stats->ms_total = elapsed(c->ev[0], c->ev[4]);
-    for (int i = 0; i < npts; ++i) {
-      const double n = 5.0 * pts[i].ny;
-      stats->lu_flops += (int64_t)(8.0 / 3.0 * n * n * n);
-      stats->assemble_bytes += (int64_t)((f32 ? 8.0 : 16.0) * B.h[i].npad * B.h[i].npad);
-    }
-    stats->npts = npts;
-    stats->nconv = nconv;
-    stats->restarts_max = cycles;
-    stats->ms_arnoldi = ms_arn;
-    for (int i = 0; i < npts; ++i) {
-      stats->op_applies += hs[i].steps;
-      // factor bytes read per application: one pass over L and U
-      stats->solve_bytes += (double)hs[i].steps * (f32 ? 8.0 : 16.0) * B.h[i].npad * B.h[i].npad;
-    }
-  }
+  if (stats) fill_stats(stats, npts, pts, B, ri, mode == STAB_MODE_MIXED, steps);
   return STAB_OK;
 }
 
@@ -703,6 +738,238 @@ int stab_solve(stab_ctx *c, int npts, const stab_point *pts, int mode, stab_z *e
     stats->ms_arnoldi += fst.ms_arnoldi;
     stats->op_applies += fst.op_applies;
     stats->solve_bytes += fst.solve_bytes;
+  }
+  return STAB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Leading mode per point
+// ---------------------------------------------------------------------------
+// A discrete mode decays into the freestream (farfield ratio below
+// STAB_FARFIELD_MAX) and is off the vorticity/entropy continuum, whose phase
+// speed omega_r / alpha_r equals the edge velocity 1; a spatial mode must also
+// travel downstream (alpha_r > 0).
+static bool lead_discrete(const stab_point &p, zd ev, double ff) {
+  if (!(ff < STAB_FARFIELD_MAX)) return false;
+  double c;
+  if (p.itype == 1) {
+    if (p.alpha.re == 0.0) return true;
+    c = ev.x / p.alpha.re;
+  } else {
+    if (!(ev.x > 0.0)) return false;   // upstream-travelling spatial branch
+    c = p.omega.re / ev.x;
+  }
+  return fabs(c - 1.0) > STAB_CONTINUUM_BAND;
+}
+
+// Larger is less stable: Im(omega) (temporal), -Im(alpha) (spatial).
+static double lead_key(const stab_point &p, zd ev) { return p.itype == 1 ? ev.y : -ev.y; }
+
+struct LeadOut {
+  stab_z *eval;
+  stab_z *evec;
+  double *resid, *ff;
+  int32_t *iters, *info;
+};
+
+// Copy column sel[i] of every point's device eigenvector block to the
+// caller's compact layout (sum of nvec per point).
+static int gather_lead_vecs(stab_ctx *c, int npts, Batch &B, const std::vector<int> &sel, stab_z *evec) {
+  std::vector<long long> doff(npts + 1, 0);
+  for (int i = 0; i < npts; ++i) doff[i + 1] = doff[i] + B.h[i].nvec;
+  HIPCHECK(c, hipMemcpyAsync(B.sel, sel.data(), sizeof(int) * npts, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(B.doff, doff.data(), sizeof(long long) * (npts + 1), hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_gather_cols, dim3(npts), dim3(256), 0, c->stream, B.pts, B.evec, B.sel, B.doff, B.lead_vec);
+  HIPCHECK(c, hipGetLastError());
+  HIPCHECK(c, hipMemcpyAsync(evec, B.lead_vec, sizeof(zd) * doff[npts], hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  // points without a selection keep zeros
+  for (int i = 0; i < npts; ++i)
+    if (sel[i] < 0) memset(evec + doff[i], 0, sizeof(stab_z) * B.h[i].nvec);
+  return STAB_OK;
+}
+
+// Krylov-Schur for the pt.nev pairs nearest pt.shift, then the leading
+// discrete pair among those with residual below 1e-10.
+static int lead_krylov(stab_ctx *c, int npts, const stab_point *pts, int mode, const LeadOut &o, stab_stats *stats) {
+  Batch B;
+  RunInfo ri;
+  int rc;
+  if ((rc = solve_run(c, npts, pts, mode, B, ri)) != STAB_OK) return rc;
+  const size_t ne = (size_t)KS_NEVMAX * npts;
+  std::vector<zd> ev(ne);
+  std::vector<double> rs(ne), ff(ne);
+  std::vector<int> inf(npts);
+  std::vector<KSState> hs(npts);
+  HIPCHECK(c, hipMemcpyAsync(ev.data(), B.eval, sizeof(zd) * ne, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(rs.data(), B.resid, sizeof(double) * ne, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(ff.data(), B.ff, sizeof(double) * ne, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(inf.data(), B.info, sizeof(int) * npts, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(hs.data(), B.state, sizeof(KSState) * npts, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  std::vector<int> sel(npts, -1), steps(npts);
+  int found = 0;
+  for (int i = 0; i < npts; ++i) {
+    steps[i] = hs[i].steps;
+    int best = -1;
+    for (int k = 0; k < pts[i].nev && !inf[i]; ++k) {
+      const size_t e = (size_t)i * KS_NEVMAX + k;
+      if (!(rs[e] < 1e-10) || !lead_discrete(pts[i], ev[e], ff[e])) continue;
+      if (best < 0 || lead_key(pts[i], ev[e]) > lead_key(pts[i], ev[(size_t)i * KS_NEVMAX + best])) best = k;
+    }
+    sel[i] = best;
+    const size_t e = (size_t)i * KS_NEVMAX + (best < 0 ? 0 : best);
+    o.eval[i].re = best < 0 ? 0.0 : ev[e].x;
+    o.eval[i].im = best < 0 ? 0.0 : ev[e].y;
+    if (o.resid) o.resid[i] = best < 0 ? 0.0 : rs[e];
+    if (o.ff) o.ff[i] = best < 0 ? 1.0 : ff[e];
+    if (o.iters) o.iters[i] = hs[i].steps;
+    o.info[i] = inf[i] ? STAB_INFO_SINGULAR : (best < 0 ? STAB_INFO_NOCONV : STAB_INFO_OK);
+    found += (best >= 0);
+  }
+  if (o.evec && (rc = gather_lead_vecs(c, npts, B, sel, o.evec)) != STAB_OK) return rc;
+  if (stats) {
+    fill_stats(stats, npts, pts, B, ri, mode == STAB_MODE_MIXED, steps);
+    stats->nconv = found;
+  }
+  return STAB_OK;
+}
+
+// Factor at pt.shift and run residual inverse iteration (k_invit).
+static int lead_invit(stab_ctx *c, int npts, const stab_point *pts, int mode, int maxit, const LeadOut &o,
+                      std::vector<int> &ok, stab_stats *stats) {
+  if (mode != STAB_MODE_PLAIN_F64 && mode != STAB_MODE_MIXED) return fail(c, STAB_E_ARG, "unknown mode %d", mode);
+  if (maxit < 1 || maxit > 1000) return fail(c, STAB_E_ARG, "maxit %d outside 1..1000", maxit);
+  for (int i = 0; i < npts; ++i) {
+    int rc = check_point(c, pts[i], i);
+    if (rc) return rc;
+  }
+  (void)hipSetDevice(c->device);
+  const bool f32 = (mode == STAB_MODE_MIXED);
+  Batch B;
+  int rc;
+  if ((rc = build_batch(c, npts, pts, f32 ? sizeof(zf) : sizeof(zd), 1, B)) != STAB_OK) return rc;
+  const size_t lds = op_lds_bytes(B.npadmax, B.nymax, 1, B.npadmax);
+  if (lds > 150 * 1024) return fail(c, STAB_E_ARG, "inverse-iteration workspace %zu B exceeds LDS", lds);
+  HIPCHECK(c, hipEventRecord(c->ev[0], c->stream));
+  if ((rc = run_prep(c, npts, B, f32)) != STAB_OK) return rc;
+  HIPCHECK(c, hipEventRecord(c->ev[1], c->stream));
+  const int cb = (B.npadmax + 7) / 8;
+  if (f32)
+    hipLaunchKernelGGL((k_assemble<float, 8>), dim3(cb, npts), dim3(256), 0, c->stream, B.pts, B.coef, B.ops, B.scales,
+                       (zf *)B.mats, (int)FORM_SHIFTED, 1);
+  else
+    hipLaunchKernelGGL((k_assemble<double, 8>), dim3(cb, npts), dim3(256), 0, c->stream, B.pts, B.coef, B.ops,
+                       B.scales, (zd *)B.mats, (int)FORM_SHIFTED, 0);
+  HIPCHECK(c, hipGetLastError());
+  HIPCHECK(c, hipEventRecord(c->ev[2], c->stream));
+  if ((rc = f32 ? launch_lu<float, 32>(c, npts, B) : launch_lu<double, 16>(c, npts, B)) != STAB_OK) return rc;
+  HIPCHECK(c, hipEventRecord(c->ev[3], c->stream));
+  // iterate to the rounding floor: with kappa ~ 1e6 (these norms) a residual
+  // of 1e-16 leaves ~1e-10 relative eigenvalue error
+  const double tol = 1e-16;
+  if (f32)
+    hipLaunchKernelGGL((k_invit<float, 32, 256>), dim3(npts), dim3(256), lds, c->stream, B.pts, (const zf *)B.mats,
+                       B.pivs, (const zf *)B.blks, B.scales, B.coef, B.ops, B.norms, maxit, tol, B.eval, B.evec,
+                       B.resid, B.ff, B.iters, B.flag, B.npadmax, B.nymax);
+  else
+    hipLaunchKernelGGL((k_invit<double, 16, 256>), dim3(npts), dim3(256), lds, c->stream, B.pts, (const zd *)B.mats,
+                       B.pivs, (const zd *)B.blks, B.scales, B.coef, B.ops, B.norms, maxit, tol, B.eval, B.evec,
+                       B.resid, B.ff, B.iters, B.flag, B.npadmax, B.nymax);
+  HIPCHECK(c, hipGetLastError());
+  HIPCHECK(c, hipEventRecord(c->ev[4], c->stream));
+  std::vector<zd> ev(npts);
+  std::vector<double> rs(npts), ff(npts);
+  std::vector<int> it(npts), fl(npts), inf(npts);
+  HIPCHECK(c, hipMemcpyAsync(ev.data(), B.eval, sizeof(zd) * npts, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(rs.data(), B.resid, sizeof(double) * npts, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(ff.data(), B.ff, sizeof(double) * npts, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(it.data(), B.iters, sizeof(int) * npts, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(fl.data(), B.flag, sizeof(int) * npts, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(inf.data(), B.info, sizeof(int) * npts, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  ok.assign(npts, 0);
+  int found = 0;
+  for (int i = 0; i < npts; ++i) {
+    o.eval[i].re = ev[i].x;
+    o.eval[i].im = ev[i].y;
+    if (o.resid) o.resid[i] = rs[i];
+    if (o.ff) o.ff[i] = ff[i];
+    if (o.iters) o.iters[i] = it[i];
+    ok[i] = !inf[i] && fl[i] == 0 && lead_discrete(pts[i], ev[i], ff[i]);
+    o.info[i] = inf[i] ? STAB_INFO_SINGULAR : (ok[i] ? STAB_INFO_OK : STAB_INFO_NOCONV);
+    found += ok[i];
+  }
+  if (o.evec) {
+    std::vector<int> sel(npts, 0);
+    if ((rc = gather_lead_vecs(c, npts, B, sel, o.evec)) != STAB_OK) return rc;
+  }
+  if (stats) {
+    RunInfo ri;
+    ri.ms_prep = elapsed(c->ev[0], c->ev[1]);
+    ri.ms_assemble = elapsed(c->ev[1], c->ev[2]);
+    ri.ms_lu = elapsed(c->ev[2], c->ev[3]);
+    ri.ms_eig = elapsed(c->ev[3], c->ev[4]);
+    ri.ms_total = elapsed(c->ev[0], c->ev[4]);
+    ri.ms_arnoldi = ri.ms_eig;
+    ri.cycles = 1;
+    ri.nconv = found;
+    fill_stats(stats, npts, pts, B, ri, f32, it);
+  }
+  return STAB_OK;
+}
+
+int stab_solve_leading(stab_ctx *c, int npts, const stab_point *pts, int mode, int method, int maxit,
+                       const stab_z *fallback_shift, stab_z *eval, stab_z *evec, double *resid, double *farfield,
+                       int32_t *iters, int32_t *info, stab_stats *stats) {
+  if (!c) return STAB_E_ARG;
+  if (npts <= 0 || !pts || !eval || !info) return fail(c, STAB_E_ARG, "npts/pts/eval/info");
+  const LeadOut o{eval, evec, resid, farfield, iters, info};
+  if (method == STAB_LEAD_KRYLOV) return lead_krylov(c, npts, pts, mode, o, stats);
+  if (method != STAB_LEAD_INVIT) return fail(c, STAB_E_ARG, "unknown method %d", method);
+  std::vector<int> ok;
+  int rc = lead_invit(c, npts, pts, mode, maxit, o, ok, stats);
+  if (rc != STAB_OK) return rc;
+  // Fallback: points whose iteration missed the residual bound or landed on a
+  // mode that is not discrete are solved by the plain FP64 Krylov-Schur path
+  // from their fallback shift.
+  std::vector<int> idx;
+  for (int i = 0; i < npts; ++i)
+    if (!ok[i]) idx.push_back(i);
+  if (stats) stats->nfallback = (int32_t)idx.size();
+  if (idx.empty()) return STAB_OK;
+  const int nf = (int)idx.size();
+  std::vector<stab_point> sub(nf);
+  for (int k = 0; k < nf; ++k) {
+    sub[k] = pts[idx[k]];
+    if (fallback_shift) sub[k].shift = fallback_shift[idx[k]];
+  }
+  std::vector<stab_z> ev(nf);
+  std::vector<double> rs(nf), ff(nf);
+  std::vector<int32_t> it(nf), inf(nf);
+  std::vector<size_t> soff(nf + 1, 0), off(npts + 1, 0);
+  for (int k = 0; k < nf; ++k) soff[k + 1] = soff[k] + (size_t)(sub[k].itype == 1 ? 5 : 10) * sub[k].ny;
+  for (int i = 0; i < npts; ++i) off[i + 1] = off[i] + (size_t)(pts[i].itype == 1 ? 5 : 10) * pts[i].ny;
+  std::vector<stab_z> vec(evec ? soff[nf] : 0);
+  const LeadOut fo{ev.data(), evec ? vec.data() : nullptr, rs.data(), ff.data(), it.data(), inf.data()};
+  stab_stats fst;
+  if ((rc = lead_krylov(c, nf, sub.data(), STAB_MODE_PLAIN_F64, fo, &fst)) != STAB_OK) return rc;
+  for (int k = 0; k < nf; ++k) {
+    const int i = idx[k];
+    eval[i] = ev[k];
+    if (resid) resid[i] = rs[k];
+    if (farfield) farfield[i] = ff[k];
+    if (iters) iters[i] += it[k];
+    info[i] = (inf[k] == STAB_INFO_OK) ? STAB_INFO_FALLBACK : inf[k];
+    if (evec) memcpy(evec + off[i], vec.data() + soff[k], sizeof(stab_z) * (soff[k + 1] - soff[k]));
+  }
+  if (stats) {
+    stats->ms_eig += fst.ms_total;
+    stats->ms_total += fst.ms_total;
+    stats->nconv += fst.nconv;
+    stats->op_applies += fst.op_applies;
+    stats->solve_bytes += fst.solve_bytes;
+    stats->restarts_max = std::max(stats->restarts_max, fst.restarts_max);
   }
   return STAB_OK;
 }

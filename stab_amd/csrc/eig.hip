@@ -669,12 +669,38 @@ __global__ __launch_bounds__(NT) void k_restart(const PointDev *__restrict__ pts
 // Final Ritz pairs: x_i = V y_i, omega_i = sigma + 1/theta_i, getevec-style
 // normalisation (first entry of largest modulus among the first 5 ny
 // components becomes 1, getevec.f90:179-189), FP64 relative residual.
+// Largest modulus over the nodes of the outer eighth of the domain (node
+// 1 .. ceil(ny/8), eta near 1, the freestream side; all five variables) of a
+// vector normalised to a largest entry of modulus 1.  Discrete modes decay
+// there, the continuous spectrum does not.
+template <int NT>
+__device__ double farfield_ratio(const PointDev &P, const zd *x, zd *red) {
+  const int nf = 5 * ((P.ny + 7) / 8);
+  double a = 0.0, b = 0.0;
+  for (int r = threadIdx.x; r < P.n; r += NT) {
+    const double v = abs2(x[r]);
+    b = fmax(b, v);
+    if (r < nf) a = fmax(a, v);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    a = fmax(a, __shfl_xor(a, off));
+    b = fmax(b, __shfl_xor(b, off));
+  }
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) { red[threadIdx.x >> 6] = mk(a, b); }
+  __syncthreads();
+  for (int w = 0; w < NT / 64; ++w) { a = fmax(a, red[w].x); b = fmax(b, red[w].y); }
+  __syncthreads();
+  return b > 0.0 ? sqrt(a / b) : 1.0;
+}
+
 template <int NT>
 __global__ __launch_bounds__(NT) void k_final(const PointDev *__restrict__ pts, const zd *__restrict__ Vall,
                                               const zd *__restrict__ small, const zd *__restrict__ coef,
                                               const double *__restrict__ ops, const double *__restrict__ norms,
                                               int m, zd *__restrict__ eval, zd *__restrict__ evec,
-                                              double *__restrict__ resid, int npadmax, int nymax, int nvecmax) {
+                                              double *__restrict__ resid, double *__restrict__ ffo, int npadmax,
+                                              int nymax, int nvecmax) {
   extern __shared__ zd lds[];
   const PointDev &P = pts[blockIdx.x];
   OpLds L(lds, P, npadmax, nymax, 1, nvecmax);
@@ -710,6 +736,10 @@ __global__ __launch_bounds__(NT) void k_final(const PointDev *__restrict__ pts, 
     __syncthreads();
     for (int r = t; r < nv; r += NT) L.z[r] = cdiv(L.z[r], scale);
     __syncthreads();
+    {
+      const double ff = farfield_ratio<NT>(P, L.z, L.red);
+      if (t == 0 && ffo) ffo[(size_t)blockIdx.x * KS_NEVMAX + i] = ff;
+    }
     // FP64 residual at the Ritz value and at the Rayleigh-quotient value of the
     // same vector; the smaller one is kept.  An operator error along the wanted
     // eigenvector (the near-null direction of A - sigma B when the shift is
@@ -809,4 +839,185 @@ __global__ __launch_bounds__(NT) void k_solve_one(const PointDev *__restrict__ p
   __syncthreads();
   shifted_solve<T, NB, NT>(P, A, ipiv, Lb, Ub, dr, dc, coef, ops, ir, L);
   for (int r = threadIdx.x; r < P.n; r += NT) x[r] = L.y[r];
+}
+
+// ---------------------------------------------------------------------------
+// Residual inverse iteration (A. Neumaier, "Residual inverse iteration for the
+// nonlinear eigenvalue problem", SIAM J. Numer. Anal. 22 (1985) 914-923) for
+// the eigenpair nearest the factored shift sigma:
+//     x <- x - L(sigma)^-1 L(lambda) x,   lambda: root of x^H L(lambda) x = 0
+// nearest the previous value, with L(l) = A - l B (temporal solver pencil) or
+// C0 + l C1 + l^2 C2 (spatial, on phi directly).  The correction
+// L(sigma)^-1 r shrinks with the residual r, so FP32 factors limit only the
+// contraction per step (|lambda - sigma| / |lambda_2 - sigma| plus about
+// kappa u32), not the attainable accuracy, which the FP64 residual sets.
+// The first step is one plain inverse-iteration step x = L(sigma)^-1 L'(sigma) r
+// from a hashed start vector.  One work-group per point.  Stops when the FP64
+// relative residual is below tol, when it has not halved for three steps while
+// below 1e-12 (the rounding floor), or after maxit corrections.  Outputs per point: lambda, the
+// vector normalised by its first entry of largest modulus (z = [phi;
+// lambda phi] for spatial points, at out_off), the residual, the farfield
+// ratio, the number of solves and a convergence flag (0, or 2 when the
+// residual stayed above 1e-13).
+// ---------------------------------------------------------------------------
+template <typename T, int NB, int NT>
+__global__ __launch_bounds__(NT) void k_invit(const PointDev *__restrict__ pts, const cx<T> *__restrict__ mats,
+                                              const int *__restrict__ pivs, const cx<T> *__restrict__ blks,
+                                              const double *__restrict__ scales, const zd *__restrict__ coef,
+                                              const double *__restrict__ ops, const double *__restrict__ norms,
+                                              int maxit, double tol, zd *__restrict__ eval, zd *__restrict__ evec,
+                                              double *__restrict__ resid, double *__restrict__ ffo,
+                                              int *__restrict__ iters, int *__restrict__ flag, int npadmax,
+                                              int nymax) {
+  extern __shared__ zd lds[];
+  const PointDev &P = pts[blockIdx.x];
+  OpLds L(lds, P, npadmax, nymax, 1, npadmax);
+  const int t = threadIdx.x, n = P.n;
+  const cx<T> *A = mats + P.mat_off;
+  const int *ipiv = pivs + P.piv_off;
+  const cx<T> *Lb = blks + P.blk_off, *Ub = nullptr;
+  const double *dr = scales + P.scale_off, *dc = dr + P.npad;
+  const bool temporal = (P.itype == 1);
+  const double nA = sqrt(norms[blockIdx.x * 4 + 0]), nB = sqrt(norms[blockIdx.x * 4 + 1]),
+               nC = sqrt(norms[blockIdx.x * 4 + 2]);
+  const zd sg = P.shift;
+  zd *x = L.z;
+  __shared__ double s_mx[NT];
+  __shared__ int s_ix[NT];
+
+  // ---- start: x = L(sigma)^-1 L'(sigma) r
+  for (int r = t; r < n; r += NT) x[r] = mk(hash01(blockIdx.x, 2 * r), hash01(blockIdx.x, 2 * r + 1));
+  __syncthreads();
+  if (temporal) {
+    pencil_apply<NT>(P, coef, ops, FORM_BM, sg, x, L.rhs, L.X1, L.X2, L.red);
+  } else {
+    pencil_apply<NT>(P, coef, ops, FORM_C1, sg, x, L.rhs, L.X1, L.X2, L.red);
+    pencil_apply<NT>(P, coef, ops, FORM_C2, sg, x, L.res, L.X1, L.X2, L.red);
+    for (int r = t; r < n; r += NT) L.rhs[r] = L.rhs[r] + (sg * 2.0) * L.res[r];
+    __syncthreads();
+  }
+  shifted_solve<T, NB, NT>(P, A, ipiv, Lb, Ub, dr, dc, coef, ops, 0, L);
+  for (int r = t; r < n; r += NT) x[r] = L.y[r];
+  __syncthreads();
+  int nsolve = 1;
+
+  zd lam = sg;
+  double rel = 1e300, best = 1e300;
+  int stall = 0;
+  for (int it = 0;; ++it) {
+    // unit 2-norm
+    {
+      const double nx = sqrt(block_norm2<NT>(x, n, L.red));
+      const double inv = (nx > 0.0) ? 1.0 / nx : 1.0;
+      for (int r = t; r < n; r += NT) x[r] = x[r] * inv;
+      __syncthreads();
+    }
+    // pencil products and the new lambda
+    zd d3[3] = {czero<double>(), czero<double>(), czero<double>()};
+    if (temporal) {
+      pencil_apply<NT>(P, coef, ops, FORM_AM, sg, x, L.y, L.X1, L.X2, L.red);
+      pencil_apply<NT>(P, coef, ops, FORM_BM, sg, x, L.rhs, L.X1, L.X2, L.red);
+      for (int r = t; r < n; r += NT) {
+        d3[0] = cfmac(d3[0], x[r], L.y[r]);
+        d3[1] = cfmac(d3[1], x[r], L.rhs[r]);
+      }
+      block_reduce<NT, 3>(d3, L.red);
+      if (cabs(d3[1]) > 0.0) lam = cdiv(d3[0], d3[1]);
+    } else {
+      pencil_apply<NT>(P, coef, ops, FORM_C0, sg, x, L.y, L.X1, L.X2, L.red);
+      pencil_apply<NT>(P, coef, ops, FORM_C1, sg, x, L.rhs, L.X1, L.X2, L.red);
+      pencil_apply<NT>(P, coef, ops, FORM_C2, sg, x, L.res, L.X1, L.X2, L.red);
+      for (int r = t; r < n; r += NT) {
+        d3[0] = cfmac(d3[0], x[r], L.y[r]);
+        d3[1] = cfmac(d3[1], x[r], L.rhs[r]);
+        d3[2] = cfmac(d3[2], x[r], L.res[r]);
+      }
+      block_reduce<NT, 3>(d3, L.red);
+      if (cabs(d3[2]) > 0.0) {
+        const zd bb = cdiv(d3[1], d3[2] * 2.0), cc = cdiv(d3[0], d3[2]);
+        const zd disc = bb * bb - cc;
+        const double rr = cabs(disc);
+        const double sr = sqrt(0.5 * (rr + fabs(disc.x)));
+        zd sq;
+        if (sr == 0.0) sq = czero<double>();
+        else if (disc.x >= 0) sq = mk(sr, disc.y / (2.0 * sr));
+        else sq = mk(fabs(disc.y) / (2.0 * sr), disc.y >= 0 ? sr : -sr);
+        const zd r1 = czero<double>() - bb + sq, r2 = czero<double>() - bb - sq;
+        lam = (cabs(r1 - lam) < cabs(r2 - lam)) ? r1 : r2;
+      } else if (cabs(d3[1]) > 0.0) {
+        lam = czero<double>() - cdiv(d3[0], d3[1]);
+      }
+    }
+    // residual r = L(lambda) x into rhs
+    zd part[1] = {czero<double>()};
+    if (temporal) {
+      for (int r = t; r < n; r += NT) {
+        const zd v = L.y[r] - lam * L.rhs[r];
+        L.rhs[r] = v;
+        part[0].x += abs2(v);
+      }
+    } else {
+      const zd l2 = lam * lam;
+      for (int r = t; r < n; r += NT) {
+        const zd v = L.y[r] + lam * L.rhs[r] + l2 * L.res[r];
+        L.rhs[r] = v;
+        part[0].x += abs2(v);
+      }
+    }
+    block_reduce<NT, 1>(part, L.red);
+    const double al = cabs(lam);
+    const double den = temporal ? (nA + al * nB) : (nA + al * nB + al * al * nC);
+    rel = sqrt(part[0].x) / den;             // ||x|| = 1
+    // the stall count only runs near the rounding floor: far from it the
+    // residual can grow for a step or two while the vector turns
+    if (rel < 0.5 * best) { best = rel; stall = 0; } else if (rel < 1e-12) { ++stall; }
+    if (rel < tol || it >= maxit || stall >= 3) break;
+    // correction
+    shifted_solve<T, NB, NT>(P, A, ipiv, Lb, Ub, dr, dc, coef, ops, 0, L);
+    ++nsolve;
+    for (int r = t; r < n; r += NT) x[r] = x[r] - L.y[r];
+    __syncthreads();
+  }
+  // ---- getevec normalisation: first entry of largest modulus set to 1
+  double bm = -1.0;
+  int bi = 0x7fffffff;
+  for (int r = t; r < n; r += NT) { const double a = abs2(x[r]); if (a > bm) { bm = a; bi = r; } }
+  s_mx[t] = bm; s_ix[t] = bi;
+  __syncthreads();
+  if (t == 0) {
+    for (int k = 1; k < NT; ++k)
+      if (s_mx[k] > bm || (s_mx[k] == bm && s_ix[k] < bi)) { bm = s_mx[k]; bi = s_ix[k]; }
+    s_ix[0] = bi;
+  }
+  __syncthreads();
+  const zd scale = x[s_ix[0]];
+  __syncthreads();
+  for (int r = t; r < n; r += NT) x[r] = cdiv(x[r], scale);
+  __syncthreads();
+  const double ff = farfield_ratio<NT>(P, x, L.red);
+  zd *out = evec + P.out_off;
+  for (int r = t; r < n; r += NT) {
+    out[r] = x[r];
+    if (!temporal) out[n + r] = lam * x[r];
+  }
+  if (t == 0) {
+    eval[blockIdx.x] = lam;
+    resid[blockIdx.x] = rel;
+    ffo[blockIdx.x] = ff;
+    iters[blockIdx.x] = nsolve;
+    flag[blockIdx.x] = (rel < 1e-13) ? 0 : 2;
+  }
+}
+
+// Copy column sel[p] of each point's eigenvector block (P.out_off, nvec rows,
+// ld nvec) to dst + doff[p].  grid: points; sel[p] < 0 skips the point.
+__global__ __launch_bounds__(256) void k_gather_cols(const PointDev *__restrict__ pts, const zd *__restrict__ src,
+                                                     const int *__restrict__ sel, const long long *__restrict__ doff,
+                                                     zd *__restrict__ dst) {
+  const PointDev &P = pts[blockIdx.x];
+  const int k = sel[blockIdx.x];
+  if (k < 0) return;
+  const zd *s = src + P.out_off + (size_t)k * P.nvec;
+  zd *d = dst + doff[blockIdx.x];
+  for (int r = threadIdx.x; r < P.nvec; r += 256) d[r] = s[r];
 }

@@ -144,6 +144,13 @@ def make_point(inp: Inputs, shift: complex, nev: int = 8, profile: int = 0) -> P
     return pt
 
 
+def clone_point(pt: Point) -> Point:
+    """Field-by-field copy of a Point."""
+    q = Point()
+    C.pointer(q)[0] = pt
+    return q
+
+
 @dataclasses.dataclass
 class SolveResult:
     evals: np.ndarray          # (npts, nev_max) complex, nearest the shift first
@@ -221,6 +228,37 @@ class Solver:
                                              perm.ctypes.data_as(C.POINTER(C.c_int32)),
                                              scales.ctypes.data_as(C.POINTER(C.c_double)), C.byref(info)))
         return lu, perm, scales[:npad], scales[npad:], info.value
+
+    def solve_leading(self, points: Sequence[Point], mode: int = _lib.MODE_PLAIN_F64,
+                      method: int = _lib.LEAD_KRYLOV, maxit: int = 30, fallback_shifts=None,
+                      want_vectors: bool = False):
+        """Leading (least stable discrete) eigenmode per point (stab_solve_leading)."""
+        npts = len(points)
+        arr = (Point * npts)(*points)
+        ev = np.zeros(npts, dtype=np.complex128)
+        rs = np.zeros(npts)
+        ff = np.zeros(npts)
+        it = np.zeros(npts, dtype=np.int32)
+        info = np.zeros(npts, dtype=np.int32)
+        st = Stats()
+        zp = C.POINTER(Z)
+        fb = None
+        if fallback_shifts is not None:
+            fb = np.ascontiguousarray(np.asarray(fallback_shifts, dtype=np.complex128))
+            assert fb.shape == (npts,)
+        nv = [(5 if p.itype == 1 else 10) * p.ny for p in points]
+        vec = np.zeros(sum(nv), dtype=np.complex128) if want_vectors else None
+        ip = C.POINTER(C.c_int32)
+        self._check(self.lib.stab_solve_leading(
+            self.ctx, npts, arr, mode, method, maxit, fb.ctypes.data_as(zp) if fb is not None else None,
+            ev.ctypes.data_as(zp), vec.ctypes.data_as(zp) if vec is not None else None,
+            rs.ctypes.data_as(C.POINTER(C.c_double)), ff.ctypes.data_as(C.POINTER(C.c_double)),
+            it.ctypes.data_as(ip), info.ctypes.data_as(ip), C.byref(st)))
+        vecs = None
+        if vec is not None:
+            offs = np.cumsum([0] + nv)
+            vecs = [vec[offs[i]:offs[i + 1]] for i in range(npts)]
+        return dict(evals=ev, resid=rs, farfield=ff, iters=it, info=info, evecs=vecs, stats={f: getattr(st, f) for f, _ in Stats._fields_})
 
     def copy_probe(self, nbytes: int, reps: int = 10) -> float:
         """Mean ms of one device-to-device copy of nbytes (stab_copy_probe)."""

@@ -293,3 +293,49 @@ def test_library_exports_every_declared_symbol():
     lib = ctypes.CDLL(_lib.LIB_PATH)
     for n in names:
         assert hasattr(lib, n), n
+
+
+def test_leading_mode_rule_picks_thesis_modes(ts_profile):
+    """The leading-mode rule (discrete: decays in the freestream, off the
+    c = 1 continuum; least stable) applied to the QZ spectrum (|eigenvalue|
+    < 1; beyond lie under-resolved modes of |alpha| ~ 1e6) picks the modes
+    the reference's thesis cases select with getevec -v
+    (thesis/TStest/time.ref, space.ref)."""
+    ym, vt = ts_profile
+    # temporal: A0 x = w B0 x
+    p = so.parse_input(open(golden("thesis_TStest", "temporal.inp")).read())
+    pb = so.setup(p, ym, vt, spatial=False)
+    A0, B0 = so.temporal_pencil(pb)
+    w, V = so.qz(A0, B0)
+    hdr, _ = so.parse_table(open(golden("thesis_TStest", "time.ref")).read())
+    phys = [k for k in range(len(w)) if np.isfinite(w[k]) and 1e-9 < abs(w[k]) < 1.0]   # under-resolved modes excluded
+    j = so.leading_mode(1, w, V, p.ny, alpha_r=p.alpha.real, candidates=phys)
+    assert j >= 0 and abs(w[j] - hdr["Omega"]) <= 1e-8 * abs(hdr["Omega"])
+    # the continuum modes nearest Im = 0 are rejected, and they exist above the TS mode
+    near = [k for k in phys if w[k].imag > w[j].imag]
+    assert all(not so.is_discrete(1, w[k], so.farfield_ratio(V[:, k], p.ny), alpha_r=p.alpha.real) for k in near)
+    # spatial: the companion pencil, alpha = eigenvalue
+    p = so.parse_input(open(golden("thesis_TStest", "spatial.inp")).read())
+    pb = so.setup(p, ym, vt, spatial=True)
+    C0, C1, C2 = so.spatial_pencil(pb)
+    L, M = so.spatial_linearization(C0, C1, C2)
+    w, V = so.qz(L, M)
+    hdr, _ = so.parse_table(open(golden("thesis_TStest", "space.ref")).read())
+    phys = [k for k in range(len(w)) if np.isfinite(w[k]) and 1e-9 < abs(w[k]) < 1.0]
+    j = so.leading_mode(2, w, V[:5 * p.ny], p.ny, omega_r=p.omega.real, candidates=phys)
+    assert j >= 0 and abs(w[j] - hdr["Alpha"]) <= 1e-8 * abs(hdr["Alpha"])
+
+
+def test_discrete_rule_rejects_continuum_and_growing_farfield():
+    """Closed-form cases of the rule: a vector that is largest in the outer
+    eighth is not discrete; one that decays is, unless its phase speed is on
+    the continuum."""
+    ny = 16
+    decay = np.exp(-np.linspace(0, 8, ny))[::-1].repeat(5)     # node 1 = freestream: small there
+    grow = decay[::-1].copy()
+    assert so.farfield_ratio(decay, ny) < 0.01 and so.farfield_ratio(grow, ny) == 1.0
+    assert so.is_discrete(1, 0.03 - 0.001j, so.farfield_ratio(decay, ny), alpha_r=0.1)
+    assert not so.is_discrete(1, 0.03 - 0.001j, so.farfield_ratio(grow, ny), alpha_r=0.1)
+    assert not so.is_discrete(1, 0.1 - 0.001j, so.farfield_ratio(decay, ny), alpha_r=0.1)   # c = 1
+    assert not so.is_discrete(2, 0.1 - 0.001j, so.farfield_ratio(decay, ny), omega_r=0.1)   # c = 1
+    assert not so.is_discrete(2, -0.01 - 0.9j, so.farfield_ratio(decay, ny), omega_r=0.1)   # upstream
