@@ -268,7 +268,6 @@ int launch_lu_nt(stab_ctx *c, int npts, Batch &B) {
   switch (R) {
     case 1: hipLaunchKernelGGL((k_lu<T, NB, NT, 1, 16>), dim3(npts), dim3(NT), 0, c->stream, B.pts, mats, B.pivs, blks, B.info, nullptr); break;
     case 2: hipLaunchKernelGGL((k_lu<T, NB, NT, 2, 16>), dim3(npts), dim3(NT), 0, c->stream, B.pts, mats, B.pivs, blks, B.info, nullptr); break;
-    case 3: hipLaunchKernelGGL((k_lu<T, NB, NT, 3, 16>), dim3(npts), dim3(NT), 0, c->stream, B.pts, mats, B.pivs, blks, B.info, nullptr); break;
     default: return fail(c, STAB_E_ARG, "matrix order %d too large for the LU kernel", B.npadmax);
   }
   HIPCHECK(c, hipGetLastError());
@@ -277,7 +276,8 @@ int launch_lu_nt(stab_ctx *c, int npts, Batch &B) {
 
 template <typename T, int NB>
 int launch_lu(stab_ctx *c, int npts, Batch &B) {
-  int rc = (B.npadmax <= 768) ? launch_lu_nt<T, NB, 256>(c, npts, B) : launch_lu_nt<T, NB, 512>(c, npts, B);
+  // two panel rows per thread: 6 waves up to order 768, 10 waves up to 1280
+  int rc = (B.npadmax <= 768) ? launch_lu_nt<T, NB, 384>(c, npts, B) : launch_lu_nt<T, NB, 640>(c, npts, B);
   if (rc != STAB_OK) return rc;
   hipLaunchKernelGGL((k_blkinv<T, SOLVE_BS>), dim3((B.npadmax + SOLVE_BS - 1) / SOLVE_BS, npts), dim3(SOLVE_BS), 0,
                      c->stream, B.pts, (const cx<T> *)B.mats, (cx<T> *)B.blks);
@@ -503,7 +503,7 @@ int stab_shifted_solve(stab_ctx *c, const stab_point *pt, int mode, int ir_steps
   else
     hipLaunchKernelGGL((k_assemble<double, 8>), dim3(cb, 1), dim3(256), 0, c->stream, B.pts, B.coef, B.ops,
                        B.scales, (zd *)B.mats, (int)FORM_SHIFTED, 0);
-  if ((rc = f32 ? launch_lu<float, 32>(c, 1, B) : launch_lu<double, 16>(c, 1, B)) != STAB_OK) return rc;
+  if ((rc = f32 ? launch_lu<float, 16>(c, 1, B) : launch_lu<double, 8>(c, 1, B)) != STAB_OK) return rc;
   const int n = B.h[0].n;
   void *p;
   if ((rc = buf_get(c, "solve_io", sizeof(zd) * 2 * (size_t)n, &p)) != STAB_OK) return rc;
@@ -511,7 +511,7 @@ int stab_shifted_solve(stab_ctx *c, const stab_point *pt, int mode, int ir_steps
   HIPCHECK(c, hipMemcpyAsync(db, b, sizeof(zd) * n, hipMemcpyHostToDevice, c->stream));
   const size_t lds = op_lds_bytes(B.npadmax, B.nymax, 1, B.npadmax);
   if (f32)
-    hipLaunchKernelGGL((k_solve_one<float, 32, 256>), dim3(1), dim3(256), lds, c->stream, B.pts, (const zf *)B.mats,
+    hipLaunchKernelGGL((k_solve_one<float, 16, 256>), dim3(1), dim3(256), lds, c->stream, B.pts, (const zf *)B.mats,
                        B.pivs, (const zf *)B.blks, B.scales, B.coef, B.ops, db, dx, ir_steps, B.npadmax, B.nymax);
   else
     hipLaunchKernelGGL((k_solve_one<double, 16, 256>), dim3(1), dim3(256), lds, c->stream, B.pts, (const zd *)B.mats,
@@ -543,7 +543,7 @@ int stab_lu_factors(stab_ctx *c, const stab_point *pt, int mode, stab_z *lu, int
   else
     hipLaunchKernelGGL((k_assemble<double, 8>), dim3(cb, 1), dim3(256), 0, c->stream, B.pts, B.coef, B.ops,
                        B.scales, (zd *)B.mats, (int)FORM_SHIFTED, 0);
-  if ((rc = f32 ? launch_lu<float, 32>(c, 1, B) : launch_lu<double, 16>(c, 1, B)) != STAB_OK) return rc;
+  if ((rc = f32 ? launch_lu<float, 16>(c, 1, B) : launch_lu<double, 8>(c, 1, B)) != STAB_OK) return rc;
   const int np = B.h[0].npad;
   std::vector<int> pv(2 * (size_t)np);
   std::vector<double> sc(2 * (size_t)np);
@@ -613,10 +613,10 @@ static int solve_run(stab_ctx *c, int npts, const stab_point *pts, int mode, Bat
                        B.scales, (zd *)B.mats, (int)FORM_SHIFTED, 0);
   HIPCHECK(c, hipGetLastError());
   HIPCHECK(c, hipEventRecord(c->ev[2], c->stream));
-  if ((rc = f32 ? launch_lu<float, 32>(c, npts, B) : launch_lu<double, 16>(c, npts, B)) != STAB_OK) return rc;
+  if ((rc = f32 ? launch_lu<float, 16>(c, npts, B) : launch_lu<double, 8>(c, npts, B)) != STAB_OK) return rc;
   HIPCHECK(c, hipEventRecord(c->ev[3], c->stream));
-  if ((rc = f32 ? run_eig<float, 32>(c, npts, B, kp, ri.cycles, ri.nconv, ri.ms_arnoldi)
-                : run_eig<double, 16>(c, npts, B, kp, ri.cycles, ri.nconv, ri.ms_arnoldi)) != STAB_OK)
+  if ((rc = f32 ? run_eig<float, 16>(c, npts, B, kp, ri.cycles, ri.nconv, ri.ms_arnoldi)
+                : run_eig<double, 8>(c, npts, B, kp, ri.cycles, ri.nconv, ri.ms_arnoldi)) != STAB_OK)
     return rc;
   HIPCHECK(c, hipEventRecord(c->ev[4], c->stream));
   HIPCHECK(c, hipEventSynchronize(c->ev[4]));
@@ -863,13 +863,13 @@ static int lead_invit(stab_ctx *c, int npts, const stab_point *pts, int mode, in
                        B.scales, (zd *)B.mats, (int)FORM_SHIFTED, 0);
   HIPCHECK(c, hipGetLastError());
   HIPCHECK(c, hipEventRecord(c->ev[2], c->stream));
-  if ((rc = f32 ? launch_lu<float, 32>(c, npts, B) : launch_lu<double, 16>(c, npts, B)) != STAB_OK) return rc;
+  if ((rc = f32 ? launch_lu<float, 16>(c, npts, B) : launch_lu<double, 8>(c, npts, B)) != STAB_OK) return rc;
   HIPCHECK(c, hipEventRecord(c->ev[3], c->stream));
   // iterate to the rounding floor: with kappa ~ 1e6 (these norms) a residual
   // of 1e-16 leaves ~1e-10 relative eigenvalue error
   const double tol = 1e-16;
   if (f32)
-    hipLaunchKernelGGL((k_invit<float, 32, 256>), dim3(npts), dim3(256), lds, c->stream, B.pts, (const zf *)B.mats,
+    hipLaunchKernelGGL((k_invit<float, 16, 256>), dim3(npts), dim3(256), lds, c->stream, B.pts, (const zf *)B.mats,
                        B.pivs, (const zf *)B.blks, B.scales, B.coef, B.ops, B.norms, maxit, tol, B.eval, B.evec,
                        B.resid, B.ff, B.iters, B.flag, B.npadmax, B.nymax);
   else

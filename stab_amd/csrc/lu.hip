@@ -1,23 +1,72 @@
-// Batched blocked complex LU with partial pivoting, one work-group per matrix.
+// Batched blocked complex LU with partial pivoting, one work-group per matrix
+// (the getrf structure: factor a block column, swap, triangular solve, update).
 //
-// Right-looking, panel width NB.  The panel rows live in registers of the
-// threads that own them (row r belongs to thread r % NT), so the pivot search,
-// the row interchanges inside the panel and the rank-1 updates never touch
-// memory.  The trailing matrix is then processed in strips of SW columns:
-// the panel's row interchanges are applied to the strip through LDS, U12 is
-// formed as inv(L11) * A12 (inv(L11) computed once per panel), and the strip
-// is updated with A22 -= L21 U12, L21 read from registers and U12 broadcast
-// from LDS.  The interchanges are applied to the columns left of the panel
-// too, so P A = L U with P given by perm[] as well as by ipiv[].
-// Outputs: L and U in place, ipiv (absolute row indices, as ZGETRF) and perm
-// (ipiv + n).  k_blkinv then inverts the 64 x 64 diagonal blocks used by the
-// blocked triangular solves.
+// Columns are taken in blocks of KB = 64 (the last may be 32 wide).  Inside a
+// block, panels of NB columns (16 in FP32, 8 in FP64) are factored with their rows in registers
+// of the threads that own them (row r belongs to thread r % NT), so the pivot
+// search, the row interchanges inside the panel and the rank-1 updates never
+// touch memory; each panel then updates the rest of its block column (all
+// rows) in strips of SW columns, applying its interchanges through LDS and
+// forming its rows of U as inv(L11) * A12.  When the block column is done:
+//   * its net row permutation is applied to the columns left of it and to the
+//     trailing columns (one pass each),
+//   * U12 = inv(L11) A12 by blocked forward substitution in LDS, with the
+//     16 x 16 inverses kept from the panels, and
+//   * A22 -= L21 U12 with depth KB on the matrix cores, complex arithmetic as
+//     2 x 2 real blocks, [Cr; Ci] -= [[Lr, -Li], [Li, Lr]] [Ur; Ui], i.e.
+//     four real MFMA products per tile, computed transposed (the MFMA row index
+//     runs over matrix columns) so that each accumulator register of 16 lanes
+//     covers 16 consecutive matrix rows and loads/stores coalesce.
+// The trailing matrix is read and written once per 64 columns.  P A = L U
+// with P given by perm[] as well as by ipiv[] (L is stored in final row
+// order).  Outputs: L and U in place, ipiv (absolute row indices, as ZGETRF),
+// perm (ipiv + n) and info.  k_blkinv then inverts the 64 x 64 diagonal blocks
+// used by the blocked triangular solves.
 #include "common.h"
+
+// 16 x 16 x 4 MFMA: D[m][n] += sum_k A[m][k] B[k][n]; lane l supplies
+// A[l % 16][l / 16] and B[l / 16][l % 16].  Accumulator register i of lane l
+// holds D[m][l % 16] with m = 4 (l / 16) + i (f32) or m = 4 i + l / 16 (f64).
+template <typename T> struct Mfma;
+template <> struct Mfma<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ inline v4 mac(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ inline int row_of(int lane, int i) { return 4 * i + (lane >> 4); }
+};
+template <> struct Mfma<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ inline v4 mac(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ inline int row_of(int lane, int i) { return 4 * (lane >> 4) + i; }
+};
+
+// Columns of one LDS tile of U12 in the MFMA update (f64: 32, f32: 64).
+template <typename T> __host__ __device__ constexpr int lu_gw() { return sizeof(T) == 8 ? 32 : 64; }
 
 // Elements of the diagonal-block inverse storage of one point (see k_blkinv).
 __host__ __device__ inline long long blk_elems(int npad, int bs) {
   const long long nblk = (npad + bs - 1) / bs;
   return 2LL * nblk * bs * bs;
+}
+
+// Net permutation of the interchanges ipiv[k0 .. k0+w) (absolute rows, each
+// >= its step): touched rows srow[e] receive the input row sorig[e];
+// returns the number of touched rows (<= 2 w).  One thread.
+__device__ inline int net_perm(const int *ipiv, int k0, int w, int *srow, int *sorig) {
+  int ns = 0;
+  for (int c = 0; c < w; ++c) {
+    const int a = k0 + c, b = ipiv[c];
+    if (a == b) continue;
+    int ia = -1, ib = -1;
+    for (int e = 0; e < ns; ++e) { if (srow[e] == a) ia = e; if (srow[e] == b) ib = e; }
+    if (ia < 0) { ia = ns; srow[ns] = a; sorig[ns] = a; ++ns; }
+    if (ib < 0) { ib = ns; srow[ns] = b; sorig[ns] = b; ++ns; }
+    const int tmp = sorig[ia]; sorig[ia] = sorig[ib]; sorig[ib] = tmp;
+  }
+  return ns;
 }
 
 template <typename T, int NB, int NT, int R, int SW>
@@ -32,236 +81,383 @@ __global__ __launch_bounds__(NT) void k_lu(const PointDev *__restrict__ pts, cx<
   int *ipiv = pivs + P.piv_off;
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   constexpr int NW = NT / 64;
+  constexpr int KB = 64, GW = lu_gw<T>(), UP = GW + 1, NPB = KB / NB;
+  static_assert(KB % NB == 0, "panel width must divide the block width");
 
   __shared__ C s_prow[NB], s_crow[NB];
   __shared__ T s_rv[NW];
   __shared__ int s_ri[NW];
   __shared__ C s_blk[NB * NB];
-  __shared__ C s_linv[NB * NB];
-  __shared__ int s_piv[NB];
-  __shared__ int s_srow[2 * NB], s_sorig[2 * NB];
-  __shared__ int s_topsrc[NB], s_outrow[NB], s_outsrc[NB];
+  __shared__ C s_linv[NPB][NB * NB];      // inv(L11) of each panel of the block
+  __shared__ int s_piv[KB];
+  __shared__ int s_srow[2 * KB], s_sorig[2 * KB];
+  __shared__ int s_topsrc[KB], s_outrow[KB], s_outsrc[KB];
   __shared__ int s_nout, s_ns;
   __shared__ C s_stage[2 * NB * SW];
   __shared__ C s_u12[NB * SW];
+  __shared__ C s_u[KB * UP];
 
   C Pn[R][NB];
   int myinfo = 0;
 
-  for (int k0 = 0; k0 < n; k0 += NB) {
-    // ---- load the panel rows owned by this thread
-#pragma unroll
-    for (int j = 0; j < R; ++j) {
-      const int r = t + j * NT;
-#pragma unroll
-      for (int c = 0; c < NB; ++c)
-        Pn[j][c] = (r >= k0 && r < n) ? A[(size_t)(k0 + c) * lda + r] : czero<T>();
-    }
-    // ---- factor the panel.  The column loop is not unrolled; every register
-    // access below uses compile-time indices with predicates, so Pn stays in
-    // VGPRs.
-#pragma unroll 1
-    for (int c = 0; c < NB; ++c) {
-      const int col = k0 + c;
-      T best = (T)-1;
-      int bi = 0x7fffffff;
+  for (int K0 = 0; K0 < n; K0 += KB) {
+    const int kb = min(KB, n - K0);       // 64, or 32 for the last block (n is a multiple of 32)
+    const int kend = K0 + kb;
+    for (int k0 = K0; k0 < kend; k0 += NB) {
+      const int pb = (k0 - K0) / NB;
+      // ---- load the panel rows owned by this thread
 #pragma unroll
       for (int j = 0; j < R; ++j) {
         const int r = t + j * NT;
-        C pc = czero<T>();
 #pragma unroll
-        for (int cc = 0; cc < NB; ++cc) if (cc == c) pc = Pn[j][cc];
-        if (r >= col && r < n) {
-          const T v = cabs1(pc);
-          if (v > best) { best = v; bi = r; }
+        for (int c = 0; c < NB; ++c)
+          Pn[j][c] = (r >= k0 && r < n) ? A[(size_t)(k0 + c) * lda + r] : czero<T>();
+      }
+      // ---- factor the panel.  The column loop is not unrolled; every register
+      // access below uses compile-time indices with predicates, so Pn stays in
+      // VGPRs.
+#pragma unroll 1
+      for (int c = 0; c < NB; ++c) {
+        const int col = k0 + c;
+        T best = (T)-1;
+        int bi = 0x7fffffff;
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+          const int r = t + j * NT;
+          C pc = czero<T>();
+#pragma unroll
+          for (int cc = 0; cc < NB; ++cc) if (cc == c) pc = Pn[j][cc];
+          if (r >= col && r < n) {
+            const T v = cabs1(pc);
+            if (v > best) { best = v; bi = r; }
+          }
         }
-      }
 #pragma unroll
-      for (int off = 32; off > 0; off >>= 1) {
-        const T ov = __shfl_xor(best, off);
-        const int oi = __shfl_xor(bi, off);
-        if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
-      }
-      if (lane == 0) { s_rv[wv] = best; s_ri[wv] = bi; }
-      __syncthreads();
-      best = s_rv[0]; bi = s_ri[0];
-#pragma unroll
-      for (int w = 1; w < NW; ++w) {
-        if (s_rv[w] > best || (s_rv[w] == best && s_ri[w] < bi)) { best = s_rv[w]; bi = s_ri[w]; }
-      }
-      // no finite candidate (NaN column): keep the diagonal so every index stays in range
-      const int p = (bi >= col && bi < n) ? bi : col;
-      const int pj = p / NT, cj = col / NT;
-#pragma unroll
-      for (int j = 0; j < R; ++j) {
-        if (j == pj && (p % NT) == t) {
-#pragma unroll
-          for (int cc = 0; cc < NB; ++cc) s_prow[cc] = Pn[j][cc];
+        for (int off = 32; off > 0; off >>= 1) {
+          const T ov = __shfl_xor(best, off);
+          const int oi = __shfl_xor(bi, off);
+          if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
         }
-        if (j == cj && (col % NT) == t) {
+        if (lane == 0) { s_rv[wv] = best; s_ri[wv] = bi; }
+        __syncthreads();
+        best = s_rv[0]; bi = s_ri[0];
 #pragma unroll
-          for (int cc = 0; cc < NB; ++cc) s_crow[cc] = Pn[j][cc];
+        for (int w = 1; w < NW; ++w) {
+          if (s_rv[w] > best || (s_rv[w] == best && s_ri[w] < bi)) { best = s_rv[w]; bi = s_ri[w]; }
         }
-      }
-      __syncthreads();
-      if (p != col) {
+        // no finite candidate (NaN column): keep the diagonal so every index stays in range
+        const int p = (bi >= col && bi < n) ? bi : col;
+        const int pj = p / NT, cj = col / NT;
 #pragma unroll
         for (int j = 0; j < R; ++j) {
           if (j == pj && (p % NT) == t) {
 #pragma unroll
-            for (int cc = 0; cc < NB; ++cc) Pn[j][cc] = s_crow[cc];
+            for (int cc = 0; cc < NB; ++cc) s_prow[cc] = Pn[j][cc];
           }
           if (j == cj && (col % NT) == t) {
 #pragma unroll
-            for (int cc = 0; cc < NB; ++cc) Pn[j][cc] = s_prow[cc];
+            for (int cc = 0; cc < NB; ++cc) s_crow[cc] = Pn[j][cc];
           }
         }
-      }
-      const C piv = s_prow[c];
-      if (t == 0) s_piv[c] = p;
-      if (piv.x == (T)0 && piv.y == (T)0) {
-        if (myinfo == 0) myinfo = col + 1;
-      } else {
-        const C inv = cdiv(mk<T>(1, 0), piv);
+        __syncthreads();
+        if (p != col) {
 #pragma unroll
-        for (int j = 0; j < R; ++j) {
-          const int r = t + j * NT;
-          if (r > col && r < n) {
-            C l = czero<T>();
+          for (int j = 0; j < R; ++j) {
+            if (j == pj && (p % NT) == t) {
 #pragma unroll
-            for (int cc = 0; cc < NB; ++cc) if (cc == c) l = Pn[j][cc];
-            l = l * inv;
-#pragma unroll
-            for (int cc = 0; cc < NB; ++cc) {
-              if (cc == c) Pn[j][cc] = l;
-              else if (cc > c) Pn[j][cc] = cfms(Pn[j][cc], l, s_prow[cc]);
+              for (int cc = 0; cc < NB; ++cc) Pn[j][cc] = s_crow[cc];
             }
+            if (j == cj && (col % NT) == t) {
+#pragma unroll
+              for (int cc = 0; cc < NB; ++cc) Pn[j][cc] = s_prow[cc];
+            }
+          }
+        }
+        const C piv = s_prow[c];
+        if (t == 0) s_piv[col - K0] = p;
+        if (piv.x == (T)0 && piv.y == (T)0) {
+          if (myinfo == 0) myinfo = col + 1;
+        } else {
+          const C inv = cdiv(mk<T>(1, 0), piv);
+#pragma unroll
+          for (int j = 0; j < R; ++j) {
+            const int r = t + j * NT;
+            if (r > col && r < n) {
+              C l = czero<T>();
+#pragma unroll
+              for (int cc = 0; cc < NB; ++cc) if (cc == c) l = Pn[j][cc];
+              l = l * inv;
+#pragma unroll
+              for (int cc = 0; cc < NB; ++cc) {
+                if (cc == c) Pn[j][cc] = l;
+                else if (cc > c) Pn[j][cc] = cfms(Pn[j][cc], l, s_prow[cc]);
+              }
+            }
+          }
+        }
+        __syncthreads();
+      }
+      // ---- write the factored panel back; stage the diagonal block
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        const int r = t + j * NT;
+        if (r >= k0 && r < n) {
+#pragma unroll
+          for (int c = 0; c < NB; ++c) A[(size_t)(k0 + c) * lda + r] = Pn[j][c];
+          if (r < k0 + NB) {
+#pragma unroll
+            for (int c = 0; c < NB; ++c) s_blk[(r - k0) * NB + c] = Pn[j][c];
           }
         }
       }
       __syncthreads();
-    }
-    // ---- write the factored panel back; stage the diagonal block
-#pragma unroll
-    for (int j = 0; j < R; ++j) {
-      const int r = t + j * NT;
-      if (r >= k0 && r < n) {
-#pragma unroll
-        for (int c = 0; c < NB; ++c) A[(size_t)(k0 + c) * lda + r] = Pn[j][c];
-        if (r < k0 + NB) {
-#pragma unroll
-          for (int c = 0; c < NB; ++c) s_blk[(r - k0) * NB + c] = Pn[j][c];
+      // ---- inv(L11) (unit lower), one column per thread
+      if (t < NB) {
+        const int c = t;
+        C *Li = s_linv[pb];
+        for (int i = 0; i < NB; ++i) {
+          C s = (i == c) ? mk<T>(1, 0) : czero<T>();
+          if (i > c)
+            for (int k = c; k < i; ++k) s = cfms(s, s_blk[i * NB + k], Li[k * NB + c]);
+          Li[i * NB + c] = (i < c) ? czero<T>() : s;
         }
       }
-    }
-    __syncthreads();
-    // ---- inv(L11) (unit lower), one column per thread, for U12 = inv(L11) A12
-    if (t < NB) {
-      const int c = t;
-      for (int i = 0; i < NB; ++i) {
-        C s = (i == c) ? mk<T>(1, 0) : czero<T>();
-        if (i > c)
-          for (int k = c; k < i; ++k) s = cfms(s, s_blk[i * NB + k], s_linv[k * NB + c]);
-        s_linv[i * NB + c] = (i < c) ? czero<T>() : s;
+      __syncthreads();
+      if (t < NB) ipiv[k0 + t] = s_piv[k0 - K0 + t];
+      // ---- net permutation of the panel's interchanges: rows of the panel
+      // top and the rows below it that change
+      if (t == 0) {
+        const int ns = net_perm(s_piv + (k0 - K0), k0, NB, s_srow, s_sorig);
+        for (int c = 0; c < NB; ++c) s_topsrc[c] = k0 + c;
+        int no = 0;
+        for (int e = 0; e < ns; ++e) {
+          const int row = s_srow[e];
+          if (row < k0 + NB) s_topsrc[row - k0] = s_sorig[e];
+          else if (s_sorig[e] != row) { s_outrow[no] = row; s_outsrc[no] = s_sorig[e]; ++no; }
+        }
+        s_nout = no;
+        s_ns = ns;
       }
-    }
-    __syncthreads();
-    if (t < NB) ipiv[k0 + t] = s_piv[t];
-    // ---- net permutation of the panel's interchanges (rows touched, origins)
+      __syncthreads();
+      // ---- the rest of the block column, in strips of SW columns
+      const int nout = s_nout, nslot = NB + nout;
+      const C *Li = s_linv[pb];
+      for (int c0 = k0 + NB; c0 < kend; c0 += SW) {
+        const int sw = min(SW, kend - c0);      // strips never cross the block end
+        for (int e = t; e < nslot * sw; e += NT) {
+          const int slot = e % nslot, cc = e / nslot;
+          const int src = (slot < NB) ? s_topsrc[slot] : s_outsrc[slot - NB];
+          s_stage[slot * SW + cc] = A[(size_t)(c0 + cc) * lda + src];
+        }
+        __syncthreads();
+        for (int e = t; e < nout * sw; e += NT) {
+          const int o = e % nout, cc = e / nout;
+          A[(size_t)(c0 + cc) * lda + s_outrow[o]] = s_stage[(NB + o) * SW + cc];
+        }
+        for (int e = t; e < NB * sw; e += NT) {
+          const int i = e / sw, cc = e - i * sw;
+          C u = czero<T>();
+          for (int k = 0; k <= i; ++k) u = cfma(u, Li[i * NB + k], s_stage[k * SW + cc]);
+          s_u12[i * SW + cc] = u;
+          A[(size_t)(c0 + cc) * lda + k0 + i] = u;
+        }
+        __syncthreads();
+        // A22 strip -= L21 U12: R rows x G columns of independent accumulators
+        constexpr int G = (R >= 2) ? 2 : 4;
+#pragma unroll 1
+        for (int cg = 0; cg < sw; cg += G) {
+          C acc[R][G];
+#pragma unroll
+          for (int j = 0; j < R; ++j) {
+            const int r = t + j * NT;
+            const bool live = (r >= k0 + NB && r < n);
+#pragma unroll
+            for (int q = 0; q < G; ++q) acc[j][q] = live ? A[(size_t)(c0 + cg + q) * lda + r] : czero<T>();
+          }
+#pragma unroll
+          for (int k = 0; k < NB; ++k) {
+            C u[G];
+#pragma unroll
+            for (int q = 0; q < G; ++q) u[q] = s_u12[k * SW + cg + q];
+#pragma unroll
+            for (int j = 0; j < R; ++j)
+#pragma unroll
+              for (int q = 0; q < G; ++q) acc[j][q] = cfms(acc[j][q], Pn[j][k], u[q]);
+          }
+#pragma unroll
+          for (int j = 0; j < R; ++j) {
+            const int r = t + j * NT;
+            if (r >= k0 + NB && r < n) {
+#pragma unroll
+              for (int q = 0; q < G; ++q) A[(size_t)(c0 + cg + q) * lda + r] = acc[j][q];
+            }
+          }
+        }
+        __syncthreads();
+      }
+      // ---- the panel's interchanges on the block's columns left of it, so the
+      // block's L is in one row order when the block is done
+      const int ns = s_ns;
+      if (ns > 0 && k0 > K0) {
+        for (int c0 = K0; c0 < k0; c0 += SW) {
+          const int w = min(SW, k0 - c0);
+          for (int e = t; e < ns * w; e += NT) {
+            const int slot = e % ns, cc = e / ns;
+            s_stage[slot * SW + cc] = A[(size_t)(c0 + cc) * lda + s_sorig[slot]];
+          }
+          __syncthreads();
+          for (int e = t; e < ns * w; e += NT) {
+            const int slot = e % ns, cc = e / ns;
+            A[(size_t)(c0 + cc) * lda + s_srow[slot]] = s_stage[slot * SW + cc];
+          }
+          __syncthreads();
+        }
+      }
+    }  // panels of the block
+
+    // ---- net permutation of the whole block: top rows K0 .. kend take the
+    // input rows s_topsrc[], rows below take s_outsrc[] (always input rows of
+    // the block top); s_srow/s_sorig list every touched row
     if (t == 0) {
-      int ns = 0;
-      for (int c = 0; c < NB; ++c) {
-        const int a = k0 + c, b = s_piv[c];
-        if (a == b) continue;
-        int ia = -1, ib = -1;
-        for (int e = 0; e < ns; ++e) { if (s_srow[e] == a) ia = e; if (s_srow[e] == b) ib = e; }
-        if (ia < 0) { ia = ns; s_srow[ns] = a; s_sorig[ns] = a; ++ns; }
-        if (ib < 0) { ib = ns; s_srow[ns] = b; s_sorig[ns] = b; ++ns; }
-        const int tmp = s_sorig[ia]; s_sorig[ia] = s_sorig[ib]; s_sorig[ib] = tmp;
-      }
-      for (int c = 0; c < NB; ++c) s_topsrc[c] = k0 + c;
+      const int ns = net_perm(s_piv, K0, kb, s_srow, s_sorig);
+      for (int c = 0; c < kb; ++c) s_topsrc[c] = K0 + c;
       int no = 0;
       for (int e = 0; e < ns; ++e) {
         const int row = s_srow[e];
-        if (row < k0 + NB) s_topsrc[row - k0] = s_sorig[e];
+        if (row < kend) s_topsrc[row - K0] = s_sorig[e];
         else if (s_sorig[e] != row) { s_outrow[no] = row; s_outsrc[no] = s_sorig[e]; ++no; }
       }
       s_nout = no;
       s_ns = ns;
     }
     __syncthreads();
-    // ---- trailing strips
-    const int nout = s_nout, nslot = NB + nout;
-    for (int c0 = k0 + NB; c0 < n; c0 += SW) {
-      for (int e = t; e < nslot * SW; e += NT) {
-        const int slot = e % nslot, cc = e / nslot;
-        const int src = (slot < NB) ? s_topsrc[slot] : s_outsrc[slot - NB];
-        s_stage[slot * SW + cc] = A[(size_t)(c0 + cc) * lda + src];
+    const int bns = s_ns, bnout = s_nout;
+    // ---- left columns [0, K0): move the touched rows, 16 columns at a time,
+    // through LDS (s_u is idle here; bns * 16 <= 2 KB * 16 entries fit)
+    if (bns > 0 && K0 > 0) {
+      for (int c0 = 0; c0 < K0; c0 += 16) {
+        for (int e = t; e < bns * 16; e += NT) s_u[e] = A[(size_t)(c0 + e / bns) * lda + s_sorig[e % bns]];
+        __syncthreads();
+        for (int e = t; e < bns * 16; e += NT) A[(size_t)(c0 + e / bns) * lda + s_srow[e % bns]] = s_u[e];
+        __syncthreads();
       }
-      __syncthreads();
-      for (int e = t; e < nout * SW; e += NT) {
-        const int o = e % nout, cc = e / nout;
-        A[(size_t)(c0 + cc) * lda + s_outrow[o]] = s_stage[(NB + o) * SW + cc];
-      }
-      for (int e = t; e < NB * SW; e += NT) {
-        const int i = e / SW, cc = e - i * SW;
-        C u = czero<T>();
-        for (int k = 0; k <= i; ++k) u = cfma(u, s_linv[i * NB + k], s_stage[k * SW + cc]);
-        s_u12[i * SW + cc] = u;
-        A[(size_t)(c0 + cc) * lda + k0 + i] = u;
-      }
-      __syncthreads();
-      // A22 strip -= L21 U12: R rows x G columns of independent accumulators
-      constexpr int G = (R >= 2) ? 2 : 4;
-#pragma unroll 1
-      for (int cg = 0; cg < SW; cg += G) {
-        C acc[R][G];
-#pragma unroll
-        for (int j = 0; j < R; ++j) {
-          const int r = t + j * NT;
-          const bool live = (r >= k0 + NB && r < n);
-#pragma unroll
-          for (int q = 0; q < G; ++q) acc[j][q] = live ? A[(size_t)(c0 + cg + q) * lda + r] : czero<T>();
+    }
+    // ---- trailing columns [kend, n), GW at a time: permute, solve for U12,
+    // then A22 -= L21 U12 on the matrix cores
+    if (kend < n) {
+      typedef typename Mfma<T>::v4 v4;
+      constexpr int CT = GW / 16, RT = 2;
+      for (int c0 = kend; c0 < n; c0 += GW) {
+        const int gw = min(GW, n - c0);
+        // stage the block rows of the strip in final order (some come from
+        // below the block); then the rows below that change receive their
+        // input rows, which lie in the block top and are not written before
+        // U12 is stored
+        for (int e = t; e < kb * gw; e += NT) {
+          const int k = e % kb, j = e / kb;
+          s_u[k * UP + j] = A[(size_t)(c0 + j) * lda + s_topsrc[k]];
         }
-#pragma unroll
-        for (int k = 0; k < NB; ++k) {
-          C u[G];
-#pragma unroll
-          for (int q = 0; q < G; ++q) u[q] = s_u12[k * SW + cg + q];
-#pragma unroll
-          for (int j = 0; j < R; ++j)
-#pragma unroll
-            for (int q = 0; q < G; ++q) acc[j][q] = cfms(acc[j][q], Pn[j][k], u[q]);
+        __syncthreads();
+        for (int e = t; e < bnout * gw; e += NT) {
+          const int o = e % bnout, j = e / bnout;
+          A[(size_t)(c0 + j) * lda + s_outrow[o]] = A[(size_t)(c0 + j) * lda + s_outsrc[o]];
         }
+        // U12 = inv(L11) A12 by 16-row blocks: apply the panel's inverse,
+        // then eliminate the block from the rows below it
+        for (int b = 0; b < kb / NB; ++b) {
+          const C *Lb = s_linv[b];
+          constexpr int MAXU = (NB * GW + NT - 1) / NT;
+          C u[MAXU];
 #pragma unroll
-        for (int j = 0; j < R; ++j) {
-          const int r = t + j * NT;
-          if (r >= k0 + NB && r < n) {
-#pragma unroll
-            for (int q = 0; q < G; ++q) A[(size_t)(c0 + cg + q) * lda + r] = acc[j][q];
+          for (int q = 0; q < MAXU; ++q) {
+            const int e = t + q * NT;
+            if (e < NB * gw) {
+              const int i = e / gw, j = e - (e / gw) * gw;
+              C s = czero<T>();
+              for (int k = 0; k <= i; ++k) s = cfma(s, Lb[i * NB + k], s_u[(b * NB + k) * UP + j]);
+              u[q] = s;
+            }
           }
+          __syncthreads();
+#pragma unroll
+          for (int q = 0; q < MAXU; ++q) {
+            const int e = t + q * NT;
+            if (e < NB * gw) {
+              const int i = e / gw, j = e - (e / gw) * gw;
+              s_u[(b * NB + i) * UP + j] = u[q];
+            }
+          }
+          __syncthreads();
+          const int r0 = (b + 1) * NB;
+          for (int e = t; e < (kb - r0) * gw; e += NT) {
+            const int r = r0 + e / gw, j = e - (e / gw) * gw;
+            C s = s_u[r * UP + j];
+            for (int k = 0; k < NB; ++k)
+              s = cfms(s, A[(size_t)(K0 + b * NB + k) * lda + K0 + r], s_u[(b * NB + k) * UP + j]);
+            s_u[r * UP + j] = s;
+          }
+          __syncthreads();
         }
-      }
-      __syncthreads();
-    }
-    // ---- the same interchanges on the columns left of the panel, so that L
-    // ends in the final row order (solves then start with one gather)
-    const int ns = s_ns;
-    if (ns > 0) {
-      for (int c0 = 0; c0 < k0; c0 += SW) {
-        for (int e = t; e < ns * SW; e += NT) {
-          const int slot = e % ns, cc = e / ns;
-          s_stage[slot * SW + cc] = A[(size_t)(c0 + cc) * lda + s_sorig[slot]];
+        // U12 to memory
+        for (int e = t; e < kb * gw; e += NT) {
+          const int k = e % kb, j = e / kb;
+          A[(size_t)(c0 + j) * lda + K0 + k] = s_u[k * UP + j];
+        }
+        // A22 -= L21 U12: each wave takes chunks of 32 rows (two 16-row
+        // tiles) against the gw columns of the strip
+        for (int r0 = kend + 32 * wv; r0 < n; r0 += 32 * NW) {
+          v4 dr[CT][RT], di[CT][RT];
+#pragma unroll
+          for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+              for (int i = 0; i < 4; ++i) {
+                C x = czero<T>();
+                if (ct * 16 < gw)
+                  x = A[(size_t)(c0 + ct * 16 + Mfma<T>::row_of(lane, i)) * lda + r0 + rt * 16 + (lane & 15)];
+                dr[ct][rt][i] = x.x;
+                di[ct][rt][i] = x.y;
+              }
+#pragma unroll 1
+          for (int kk = 0; kk < kb; kk += 4) {
+            const int k = kk + (lane >> 4);
+            T lr[RT], li[RT];
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt) {
+              const C x = A[(size_t)(K0 + k) * lda + r0 + rt * 16 + (lane & 15)];
+              lr[rt] = x.x;
+              li[rt] = x.y;
+            }
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct) {
+              if (ct * 16 < gw) {
+                const C uu = s_u[k * UP + ct * 16 + (lane & 15)];
+                const T nur = -uu.x, ui = uu.y, nui = -uu.y;
+#pragma unroll
+                for (int rt = 0; rt < RT; ++rt) {
+                  dr[ct][rt] = Mfma<T>::mac(nur, lr[rt], dr[ct][rt]);
+                  dr[ct][rt] = Mfma<T>::mac(ui, li[rt], dr[ct][rt]);
+                  di[ct][rt] = Mfma<T>::mac(nui, lr[rt], di[ct][rt]);
+                  di[ct][rt] = Mfma<T>::mac(nur, li[rt], di[ct][rt]);
+                }
+              }
+            }
+          }
+#pragma unroll
+          for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+              for (int i = 0; i < 4; ++i)
+                if (ct * 16 < gw)
+                  A[(size_t)(c0 + ct * 16 + Mfma<T>::row_of(lane, i)) * lda + r0 + rt * 16 + (lane & 15)] =
+                      mk<T>(dr[ct][rt][i], di[ct][rt][i]);
         }
         __syncthreads();
-        for (int e = t; e < ns * SW; e += NT) {
-          const int slot = e % ns, cc = e / ns;
-          A[(size_t)(c0 + cc) * lda + s_srow[slot]] = s_stage[slot * SW + cc];
-        }
-        __syncthreads();
       }
     }
-  }
+  }  // blocks
   // ---- final row permutation: row r of L U is row perm[r] of the input
   int *perm = ipiv + n;
   for (int r = t; r < n; r += NT) perm[r] = r;

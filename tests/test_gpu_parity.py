@@ -146,9 +146,13 @@ def test_shifted_solve_near_eigenvalue(gpu_solver, ts_profile, mode, ir, tol):
     assert berr < tol
 
 
-@pytest.mark.parametrize("mode,ny,tol", [(0, 8, 1e-14), (0, 96, 1e-13), (1, 8, 1e-5), (1, 96, 1e-5)])
+@pytest.mark.parametrize("mode,ny,tol", [(0, 8, 1e-14), (0, 70, 1e-13), (0, 96, 1e-13), (0, 128, 1e-13),
+                                         (0, 160, 1e-13), (1, 8, 1e-5), (1, 70, 1e-5), (1, 96, 1e-5),
+                                         (1, 128, 1e-5), (1, 160, 1e-5)])
 def test_lu_factors(gpu_solver, ts_profile, mode, ny, tol):
-    """P (Dr M Dc) = L U for the batched LU kernel (Dr = Dc = I in FP64)."""
+    """P (Dr M Dc) = L U for the batched LU kernel (Dr = Dc = I in FP64).
+    Orders 64 (one 64-column block), 352 (last block 32 wide), 480, 640 and
+    800 (the 640-thread variant)."""
     from stab_amd import solver as sv
     from stab_amd import _lib
     _say(f"lu factors mode {mode} ny {ny}")
