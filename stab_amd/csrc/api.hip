@@ -268,6 +268,7 @@ int launch_lu_nt(stab_ctx *c, int npts, Batch &B) {
   switch (R) {
     case 1: hipLaunchKernelGGL((k_lu<T, NB, NT, 1, 16>), dim3(npts), dim3(NT), 0, c->stream, B.pts, mats, B.pivs, blks, B.info, nullptr); break;
     case 2: hipLaunchKernelGGL((k_lu<T, NB, NT, 2, 16>), dim3(npts), dim3(NT), 0, c->stream, B.pts, mats, B.pivs, blks, B.info, nullptr); break;
+    case 3: hipLaunchKernelGGL((k_lu<T, NB, NT, 3, 16>), dim3(npts), dim3(NT), 0, c->stream, B.pts, mats, B.pivs, blks, B.info, nullptr); break;
     default: return fail(c, STAB_E_ARG, "matrix order %d too large for the LU kernel", B.npadmax);
   }
   HIPCHECK(c, hipGetLastError());
@@ -276,8 +277,9 @@ int launch_lu_nt(stab_ctx *c, int npts, Batch &B) {
 
 template <typename T, int NB>
 int launch_lu(stab_ctx *c, int npts, Batch &B) {
-  // two panel rows per thread: 6 waves up to order 768, 10 waves up to 1280
-  int rc = (B.npadmax <= 768) ? launch_lu_nt<T, NB, 384>(c, npts, B) : launch_lu_nt<T, NB, 640>(c, npts, B);
+  // 4 waves (two work-groups per CU) up to order 768, 8 waves up to 1280;
+  // at most three panel rows per thread
+  int rc = (B.npadmax <= 768) ? launch_lu_nt<T, NB, 256>(c, npts, B) : launch_lu_nt<T, NB, 512>(c, npts, B);
   if (rc != STAB_OK) return rc;
   hipLaunchKernelGGL((k_blkinv<T, SOLVE_BS>), dim3((B.npadmax + SOLVE_BS - 1) / SOLVE_BS, npts), dim3(SOLVE_BS), 0,
                      c->stream, B.pts, (const cx<T> *)B.mats, (cx<T> *)B.blks);

@@ -69,8 +69,9 @@ __device__ inline int net_perm(const int *ipiv, int k0, int w, int *srow, int *s
   return ns;
 }
 
+// Two work-groups per CU: at most 256 registers per lane in all variants.
 template <typename T, int NB, int NT, int R, int SW>
-__global__ __launch_bounds__(NT) void k_lu(const PointDev *__restrict__ pts, cx<T> *__restrict__ mats,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_lu(const PointDev *__restrict__ pts, cx<T> *__restrict__ mats,
                                            int *__restrict__ pivs, cx<T> *__restrict__ blks,
                                            int *__restrict__ info, const int *__restrict__ active) {
   typedef cx<T> C;
@@ -419,27 +420,39 @@ __global__ __launch_bounds__(NT) void k_lu(const PointDev *__restrict__ pts, cx<
                 dr[ct][rt][i] = x.x;
                 di[ct][rt][i] = x.y;
               }
-#pragma unroll 1
-          for (int kk = 0; kk < kb; kk += 4) {
-            const int k = kk + (lane >> 4);
-            T lr[RT], li[RT];
+          // L21 of the chunk is loaded LH depth-4 steps at a time before the
+          // MFMA chain that uses it (one memory latency per LH steps)
+          constexpr int LH = 4;
+          for (int kb0 = 0; kb0 < kb; kb0 += 4 * LH) {
+            T lr[LH][RT], li[LH][RT];
 #pragma unroll
-            for (int rt = 0; rt < RT; ++rt) {
-              const C x = A[(size_t)(K0 + k) * lda + r0 + rt * 16 + (lane & 15)];
-              lr[rt] = x.x;
-              li[rt] = x.y;
+            for (int st = 0; st < LH; ++st) {
+              const int k = kb0 + 4 * st + (lane >> 4);
+#pragma unroll
+              for (int rt = 0; rt < RT; ++rt) {
+                C x = czero<T>();
+                if (kb0 + 4 * st < kb) x = A[(size_t)(K0 + k) * lda + r0 + rt * 16 + (lane & 15)];
+                lr[st][rt] = x.x;
+                li[st][rt] = x.y;
+              }
             }
 #pragma unroll
-            for (int ct = 0; ct < CT; ++ct) {
-              if (ct * 16 < gw) {
-                const C uu = s_u[k * UP + ct * 16 + (lane & 15)];
-                const T nur = -uu.x, ui = uu.y, nui = -uu.y;
+            for (int st = 0; st < LH; ++st) {
+              if (kb0 + 4 * st < kb) {
+                const int k = kb0 + 4 * st + (lane >> 4);
 #pragma unroll
-                for (int rt = 0; rt < RT; ++rt) {
-                  dr[ct][rt] = Mfma<T>::mac(nur, lr[rt], dr[ct][rt]);
-                  dr[ct][rt] = Mfma<T>::mac(ui, li[rt], dr[ct][rt]);
-                  di[ct][rt] = Mfma<T>::mac(nui, lr[rt], di[ct][rt]);
-                  di[ct][rt] = Mfma<T>::mac(nur, li[rt], di[ct][rt]);
+                for (int ct = 0; ct < CT; ++ct) {
+                  if (ct * 16 < gw) {
+                    const C uu = s_u[k * UP + ct * 16 + (lane & 15)];
+                    const T nur = -uu.x, ui = uu.y, nui = -uu.y;
+#pragma unroll
+                    for (int rt = 0; rt < RT; ++rt) {
+                      dr[ct][rt] = Mfma<T>::mac(nur, lr[st][rt], dr[ct][rt]);
+                      dr[ct][rt] = Mfma<T>::mac(ui, li[st][rt], dr[ct][rt]);
+                      di[ct][rt] = Mfma<T>::mac(nui, lr[st][rt], di[ct][rt]);
+                      di[ct][rt] = Mfma<T>::mac(nur, li[st][rt], di[ct][rt]);
+                    }
+                  }
                 }
               }
             }
