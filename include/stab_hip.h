@@ -209,6 +209,45 @@ int stab_solve_leading(stab_ctx *ctx, int npts, const stab_point *pts, int mode,
                        const stab_z *fallback_shift, stab_z *eval, stab_z *evec, double *resid,
                        double *farfield, int32_t *iters, int32_t *info, stab_stats *stats);
 
+/* ---------------------------------------------------------------------------
+ * Shift predictor.  No counterpart in the reference, whose user supplies the
+ * eigenvalue guess by hand (getevec -v, thesis/TStest/run.sh:13-17): a tanh
+ * MLP that maps stability parameters to the leading eigenvalue, trained and
+ * evaluated on the GPU (FP32 master weights, BF16 matrix products on MFMA
+ * with FP32 accumulation, Adam), whose predictions serve as pt.shift for
+ * stab_solve_leading(STAB_LEAD_INVIT).
+ *   nin inputs (<= 64) -> depth hidden tanh layers of width units (multiple
+ *   of 16, <= 512) -> nout linear outputs (<= 16).  Inputs are mapped
+ *   x' = (x - in_shift) * in_scale and outputs y = y' / out_scale + out_shift.
+ * Flat FP32 parameter layout with nin_p = nin rounded up to 16 and nout_p =
+ * 16 (padded rows and columns hold zeros):
+ *   W_0 [width x nin_p], b_0 [width], then for l = 1..depth-1 W_l [width x
+ *   width], b_l [width], then W_o [nout_p x width], b_o [nout_p];
+ *   every W row-major by output unit (W[o * in + i]).
+ * Row counts passed to stab_mlp_loss_grad and batch sizes are multiples of 16.
+ * ------------------------------------------------------------------------- */
+typedef struct stab_mlp stab_mlp;
+int stab_mlp_create(stab_ctx *ctx, int nin, int width, int depth, int nout, uint64_t seed, stab_mlp **out);
+int stab_mlp_destroy(stab_mlp *mlp);
+int stab_mlp_nparams(const stab_mlp *mlp);
+int stab_mlp_get_params(stab_mlp *mlp, float *params);
+/* Also resets the Adam moments and step count. */
+int stab_mlp_set_params(stab_mlp *mlp, const float *params);
+int stab_mlp_set_scaling(stab_mlp *mlp, const float *in_shift, const float *in_scale, const float *out_shift,
+                         const float *out_scale);
+/* y (n x nout, row-major) = network(x (n x nin, row-major)). */
+int stab_mlp_predict(stab_mlp *mlp, int n, const float *x, float *y);
+/* Mean squared error over the n rows and nout outputs (in the scaled output
+ * space) and its gradient with respect to the flat parameters. */
+int stab_mlp_loss_grad(stab_mlp *mlp, int n, const float *x, const float *t, float *loss, float *grad);
+/* `steps` Adam steps (beta1 0.9, beta2 0.999, eps 1e-8, rate lr) on
+ * mini-batches of `batch` rows drawn with replacement from the n rows by a
+ * hash of (seed, step, row) -- the rows in order when batch == n.  The data
+ * set stays on the device for the call; loss_hist (NULL or steps values)
+ * receives the batch loss of every step. */
+int stab_mlp_train(stab_mlp *mlp, int n, const float *x, const float *t, int batch, int steps, float lr,
+                   uint64_t seed, float *loss_hist);
+
 /* Measurement utility with no counterpart in the reference: copies `bytes`
  * (a multiple of 16) between two device buffers `reps` times with a plain
  * 16-byte-per-thread kernel and returns the mean time of one copy.  Gives the

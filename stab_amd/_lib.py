@@ -21,7 +21,9 @@ LEAD_KRYLOV, LEAD_INVIT = 0, 1
 FARFIELD_MAX, CONTINUUM_BAND = 0.1, 0.01
 
 EXPORTS = ("stab_ctx_create", "stab_ctx_destroy", "stab_last_error", "stab_profile_set", "stab_grid",
-           "stab_pencil", "stab_solve", "stab_evec_bytes", "stab_shifted_solve", "stab_lu_factors", "stab_copy_probe", "stab_solve_leading")
+           "stab_pencil", "stab_solve", "stab_evec_bytes", "stab_shifted_solve", "stab_lu_factors", "stab_copy_probe", "stab_solve_leading",
+           "stab_mlp_create", "stab_mlp_destroy", "stab_mlp_nparams", "stab_mlp_get_params", "stab_mlp_set_params",
+           "stab_mlp_set_scaling", "stab_mlp_predict", "stab_mlp_loss_grad", "stab_mlp_train")
 
 
 class Z(C.Structure):
@@ -72,6 +74,16 @@ def load():
     lib.stab_copy_probe.argtypes = [vp, C.c_int64, C.c_int, dp]
     lib.stab_solve_leading.argtypes = [vp, C.c_int, C.POINTER(Point), C.c_int, C.c_int, C.c_int, C.POINTER(Z),
                                        C.POINTER(Z), C.POINTER(Z), dp, dp, ip, ip, C.POINTER(Stats)]
+    fp = C.POINTER(C.c_float)
+    lib.stab_mlp_create.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, C.POINTER(vp)]
+    lib.stab_mlp_destroy.argtypes = [vp]
+    lib.stab_mlp_nparams.argtypes = [vp]
+    lib.stab_mlp_get_params.argtypes = [vp, fp]
+    lib.stab_mlp_set_params.argtypes = [vp, fp]
+    lib.stab_mlp_set_scaling.argtypes = [vp, fp, fp, fp, fp]
+    lib.stab_mlp_predict.argtypes = [vp, C.c_int, fp, fp]
+    lib.stab_mlp_loss_grad.argtypes = [vp, C.c_int, fp, fp, fp, fp]
+    lib.stab_mlp_train.argtypes = [vp, C.c_int, fp, fp, C.c_int, C.c_int, C.c_float, C.c_uint64, fp]
     lib.stab_evec_bytes.restype = C.c_size_t
     for name in EXPORTS:
         lib[name].restype = lib[name].restype if name in ("stab_last_error", "stab_evec_bytes") else C.c_int

@@ -5,6 +5,7 @@
 #include "assemble.hip"
 #include "lu.hip"
 #include "eig.hip"
+#include "mlp.hip"
 
 #include <algorithm>
 #include <cmath>
@@ -362,6 +363,128 @@ __global__ __launch_bounds__(256) void k_copy16(const uint4 *__restrict__ src, u
   const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
   if (i < n16) dst[i] = src[i];
 }
+
+// ---------------------------------------------------------------------------
+// Shift predictor (mlp.hip)
+// ---------------------------------------------------------------------------
+struct stab_mlp {
+  stab_ctx *c = nullptr;
+  int nin = 0, nin_p = 0, width = 0, depth = 0, nout = 0, nout_p = 0, np = 0, t = 0;
+  std::vector<int> off_w, off_b, in_w, out_w;
+  float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr, *scl = nullptr, *loss = nullptr;
+  int *meta = nullptr;
+  int hist_cap = 0;
+  int cap_b = 0;                       // rows of the work buffers
+  std::vector<float *> act;            // depth + 1 activations, act[0] = normalised input
+  float *Y = nullptr, *dY = nullptr, *dZa = nullptr, *dZb = nullptr, *Tb = nullptr, *xin = nullptr, *tin = nullptr;
+  int cap_n = 0;                       // rows of the data set buffers
+  float *Xall = nullptr, *Tall = nullptr, *xraw = nullptr, *traw = nullptr;
+  std::vector<void *> owned;
+};
+
+namespace {
+
+int mlp_alloc(stab_mlp *M, size_t bytes, void **out) {
+  void *q = nullptr;
+  HIPCHECK(M->c, hipMalloc(&q, bytes ? bytes : 16));
+  M->owned.push_back(q);
+  *out = q;
+  return STAB_OK;
+}
+
+template <typename P>
+int mlp_get(stab_mlp *M, size_t bytes, P **out) {
+  void *q;
+  int rc = mlp_alloc(M, bytes, &q);
+  *out = (P *)q;
+  return rc;
+}
+
+void mlp_free(stab_mlp *M, void *q) {
+  if (!q) return;
+  (void)hipFree(q);
+  for (auto &o : M->owned)
+    if (o == q) o = nullptr;
+}
+
+// Work buffers for batches of up to B rows (B a multiple of 16).
+int mlp_reserve(stab_mlp *M, int B) {
+  if (B <= M->cap_b) return STAB_OK;
+  for (float *a : M->act) mlp_free(M, a);
+  mlp_free(M, M->Y); mlp_free(M, M->dY); mlp_free(M, M->dZa); mlp_free(M, M->dZb);
+  mlp_free(M, M->Tb); mlp_free(M, M->xin); mlp_free(M, M->tin);
+  int rc;
+  M->act.assign(M->depth + 1, nullptr);
+  for (int l = 0; l <= M->depth; ++l) {
+    const int w = (l == 0) ? M->nin_p : M->width;
+    if ((rc = mlp_get(M, sizeof(float) * (size_t)B * w, &M->act[l])) != STAB_OK) return rc;
+  }
+  if ((rc = mlp_get(M, sizeof(float) * (size_t)B * M->nout_p, &M->Y)) != STAB_OK) return rc;
+  if ((rc = mlp_get(M, sizeof(float) * (size_t)B * M->nout_p, &M->dY)) != STAB_OK) return rc;
+  if ((rc = mlp_get(M, sizeof(float) * (size_t)B * M->width, &M->dZa)) != STAB_OK) return rc;
+  if ((rc = mlp_get(M, sizeof(float) * (size_t)B * M->width, &M->dZb)) != STAB_OK) return rc;
+  if ((rc = mlp_get(M, sizeof(float) * (size_t)B * M->nout, &M->Tb)) != STAB_OK) return rc;
+  if ((rc = mlp_get(M, sizeof(float) * (size_t)B * M->nin, &M->xin)) != STAB_OK) return rc;
+  if ((rc = mlp_get(M, sizeof(float) * (size_t)B * M->nout, &M->tin)) != STAB_OK) return rc;
+  M->cap_b = B;
+  return STAB_OK;
+}
+
+int mlp_reserve_data(stab_mlp *M, int n) {
+  if (n <= M->cap_n) return STAB_OK;
+  mlp_free(M, M->Xall); mlp_free(M, M->Tall); mlp_free(M, M->xraw); mlp_free(M, M->traw);
+  int rc;
+  if ((rc = mlp_get(M, sizeof(float) * (size_t)n * M->nin_p, &M->Xall)) != STAB_OK) return rc;
+  if ((rc = mlp_get(M, sizeof(float) * (size_t)n * M->nout, &M->Tall)) != STAB_OK) return rc;
+  if ((rc = mlp_get(M, sizeof(float) * (size_t)n * M->nin, &M->xraw)) != STAB_OK) return rc;
+  if ((rc = mlp_get(M, sizeof(float) * (size_t)n * M->nout, &M->traw)) != STAB_OK) return rc;
+  M->cap_n = n;
+  return STAB_OK;
+}
+
+// Forward pass of B rows already normalised in act[0]; output in Y.
+void mlp_forward(stab_mlp *M, int B) {
+  hipStream_t s = M->c->stream;
+  for (int l = 0; l < M->depth; ++l) {
+    const int in = M->in_w[l], out = M->width;
+    hipLaunchKernelGGL(k_mlp_gemm<EPI_BIAS_TANH>, dim3(out / 16, B / 16), dim3(MLP_T), 0, s, in, M->act[l], in, 1,
+                       M->p + M->off_w[l], 1, in, M->act[l + 1], out, M->p + M->off_b[l], nullptr);
+  }
+  const int L = M->depth;
+  hipLaunchKernelGGL(k_mlp_gemm<EPI_BIAS>, dim3(M->nout_p / 16, B / 16), dim3(MLP_T), 0, s, M->width, M->act[L],
+                     M->width, 1, M->p + M->off_w[L], 1, M->width, M->Y, M->nout_p, M->p + M->off_b[L], nullptr);
+}
+
+// Backward pass from dY (B x nout_p) into the flat gradient g.
+void mlp_backward(stab_mlp *M, int B) {
+  hipStream_t s = M->c->stream;
+  const int L = M->depth, W = M->width, op = M->nout_p;
+  // output layer
+  hipLaunchKernelGGL(k_mlp_gemm<EPI_NONE>, dim3(W / 16, op / 16), dim3(MLP_T), 0, s, B, M->dY, 1, op, M->act[L], W,
+                     1, M->g + M->off_w[L], W, nullptr, nullptr);
+  hipLaunchKernelGGL(k_mlp_colsum, dim3((op + 255) / 256), dim3(256), 0, s, B, op, M->dY, op, M->g + M->off_b[L]);
+  hipLaunchKernelGGL(k_mlp_gemm<EPI_DTANH>, dim3(W / 16, B / 16), dim3(MLP_T), 0, s, op, M->dY, op, 1,
+                     M->p + M->off_w[L], W, 1, M->dZa, W, nullptr, M->act[L]);
+  float *dz = M->dZa, *dn = M->dZb;
+  for (int l = L - 1; l >= 0; --l) {
+    const int in = M->in_w[l];
+    hipLaunchKernelGGL(k_mlp_gemm<EPI_NONE>, dim3(in / 16, W / 16), dim3(MLP_T), 0, s, B, dz, 1, W, M->act[l], in, 1,
+                       M->g + M->off_w[l], in, nullptr, nullptr);
+    hipLaunchKernelGGL(k_mlp_colsum, dim3((W + 255) / 256), dim3(256), 0, s, B, W, dz, W, M->g + M->off_b[l]);
+    if (l > 0) {
+      hipLaunchKernelGGL(k_mlp_gemm<EPI_DTANH>, dim3(in / 16, B / 16), dim3(MLP_T), 0, s, W, dz, W, 1,
+                         M->p + M->off_w[l], in, 1, dn, in, nullptr, M->act[l]);
+      std::swap(dz, dn);
+    }
+  }
+}
+
+int mlp_check_rows(stab_mlp *M, int n) {
+  if (n <= 0 || (n % 16) != 0) return fail(M->c, STAB_E_ARG, "MLP row count %d must be a positive multiple of 16", n);
+  return STAB_OK;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -973,6 +1096,175 @@ int stab_solve_leading(stab_ctx *c, int npts, const stab_point *pts, int mode, i
     stats->solve_bytes += fst.solve_bytes;
     stats->restarts_max = std::max(stats->restarts_max, fst.restarts_max);
   }
+  return STAB_OK;
+}
+
+int stab_mlp_create(stab_ctx *c, int nin, int width, int depth, int nout, uint64_t seed, stab_mlp **out) {
+  if (!c || !out) return STAB_E_ARG;
+  *out = nullptr;
+  if (nin < 1 || nin > 64 || width < 16 || width > 512 || (width % 16) != 0 || depth < 1 || depth > 8 || nout < 1 ||
+      nout > 16)
+    return fail(c, STAB_E_ARG, "MLP sizes nin=%d width=%d depth=%d nout=%d", nin, width, depth, nout);
+  HIPCHECK(c, hipSetDevice(c->device));
+  stab_mlp *M = new stab_mlp;
+  M->c = c;
+  M->nin = nin; M->nin_p = (nin + 15) / 16 * 16; M->width = width; M->depth = depth;
+  M->nout = nout; M->nout_p = 16;
+  int off = 0;
+  std::vector<int> meta;
+  for (int l = 0; l <= depth; ++l) {
+    const int in = (l == 0) ? M->nin_p : width, o = (l == depth) ? M->nout_p : width;
+    const int rin = (l == 0) ? nin : width, rout = (l == depth) ? nout : width;
+    M->in_w.push_back(in);
+    M->out_w.push_back(o);
+    M->off_w.push_back(off);
+    meta.insert(meta.end(), {off, o, in, rout, rin, 0});
+    off += o * in;
+    M->off_b.push_back(off);
+    meta.insert(meta.end(), {off, 1, o, 1, rout, 1});
+    off += o;
+  }
+  M->np = off;
+  int rc;
+  auto bail = [&](int code) { stab_mlp_destroy(M); return code; };
+  if ((rc = mlp_get(M, sizeof(float) * M->np, &M->p)) != STAB_OK) return bail(rc);
+  if ((rc = mlp_get(M, sizeof(float) * M->np, &M->g)) != STAB_OK) return bail(rc);
+  if ((rc = mlp_get(M, sizeof(float) * M->np, &M->m)) != STAB_OK) return bail(rc);
+  if ((rc = mlp_get(M, sizeof(float) * M->np, &M->v)) != STAB_OK) return bail(rc);
+  if ((rc = mlp_get(M, sizeof(float) * (2 * M->nin_p + 32), &M->scl)) != STAB_OK) return bail(rc);
+  if ((rc = mlp_get(M, sizeof(int) * meta.size(), &M->meta)) != STAB_OK) return bail(rc);
+  M->hist_cap = 0;
+  if ((rc = mlp_get(M, sizeof(float), &M->loss)) != STAB_OK) return bail(rc);
+  std::vector<float> scl(2 * M->nin_p + 32, 0.f);
+  for (int i = 0; i < M->nin_p; ++i) scl[M->nin_p + i] = 1.f;
+  for (int o = 0; o < 16; ++o) scl[2 * M->nin_p + 16 + o] = 1.f;
+  if (hipMemcpy(M->scl, scl.data(), sizeof(float) * scl.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(M->meta, meta.data(), sizeof(int) * meta.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(M->m, 0, sizeof(float) * M->np) != hipSuccess || hipMemset(M->v, 0, sizeof(float) * M->np) != hipSuccess)
+    return bail(fail(c, STAB_E_HIP, "MLP set-up copies failed"));
+  hipLaunchKernelGGL(k_mlp_init, dim3((M->np + 255) / 256), dim3(256), 0, c->stream, M->np, M->p, M->meta,
+                     (int)(meta.size() / 6), (unsigned long long)seed);
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(fail(c, STAB_E_HIP, "MLP init failed"));
+  *out = M;
+  return STAB_OK;
+}
+
+int stab_mlp_destroy(stab_mlp *M) {
+  if (!M) return STAB_OK;
+  for (void *q : M->owned)
+    if (q) (void)hipFree(q);
+  delete M;
+  return STAB_OK;
+}
+
+int stab_mlp_nparams(const stab_mlp *M) { return M ? M->np : 0; }
+
+int stab_mlp_get_params(stab_mlp *M, float *params) {
+  if (!M || !params) return STAB_E_ARG;
+  HIPCHECK(M->c, hipMemcpy(params, M->p, sizeof(float) * M->np, hipMemcpyDeviceToHost));
+  return STAB_OK;
+}
+
+int stab_mlp_set_params(stab_mlp *M, const float *params) {
+  if (!M || !params) return STAB_E_ARG;
+  HIPCHECK(M->c, hipMemcpy(M->p, params, sizeof(float) * M->np, hipMemcpyHostToDevice));
+  HIPCHECK(M->c, hipMemset(M->m, 0, sizeof(float) * M->np));
+  HIPCHECK(M->c, hipMemset(M->v, 0, sizeof(float) * M->np));
+  M->t = 0;
+  return STAB_OK;
+}
+
+int stab_mlp_set_scaling(stab_mlp *M, const float *in_shift, const float *in_scale, const float *out_shift,
+                         const float *out_scale) {
+  if (!M || !in_shift || !in_scale || !out_shift || !out_scale) return STAB_E_ARG;
+  std::vector<float> scl(2 * M->nin_p + 32, 0.f);
+  for (int i = 0; i < M->nin_p; ++i) scl[M->nin_p + i] = 1.f;
+  for (int o = 0; o < 16; ++o) scl[2 * M->nin_p + 16 + o] = 1.f;
+  for (int i = 0; i < M->nin; ++i) { scl[i] = in_shift[i]; scl[M->nin_p + i] = in_scale[i]; }
+  for (int o = 0; o < M->nout; ++o) { scl[2 * M->nin_p + o] = out_shift[o]; scl[2 * M->nin_p + 16 + o] = out_scale[o]; }
+  HIPCHECK(M->c, hipMemcpy(M->scl, scl.data(), sizeof(float) * scl.size(), hipMemcpyHostToDevice));
+  return STAB_OK;
+}
+
+int stab_mlp_predict(stab_mlp *M, int n, const float *x, float *y) {
+  if (!M || n < 0 || (n > 0 && (!x || !y))) return STAB_E_ARG;
+  HIPCHECK(M->c, hipSetDevice(M->c->device));
+  const int CH = 4096;
+  int rc;
+  if ((rc = mlp_reserve(M, CH)) != STAB_OK) return rc;
+  for (int r0 = 0; r0 < n; r0 += CH) {
+    const int rows = std::min(CH, n - r0), B = (rows + 15) / 16 * 16;
+    HIPCHECK(M->c, hipMemsetAsync(M->xin, 0, sizeof(float) * (size_t)B * M->nin, M->c->stream));
+    HIPCHECK(M->c, hipMemcpyAsync(M->xin, x + (size_t)r0 * M->nin, sizeof(float) * (size_t)rows * M->nin,
+                                  hipMemcpyHostToDevice, M->c->stream));
+    hipLaunchKernelGGL(k_mlp_normalize, dim3(B), dim3(64), 0, M->c->stream, M->nin, M->nin_p, M->nout, M->xin,
+                       (const float *)nullptr, M->scl, M->act[0], (float *)nullptr);
+    mlp_forward(M, B);
+    hipLaunchKernelGGL(k_mlp_denormalize, dim3(B), dim3(64), 0, M->c->stream, M->nout, M->nout_p, M->nin_p, M->Y,
+                       M->scl, M->tin);
+    HIPCHECK(M->c, hipGetLastError());
+    HIPCHECK(M->c, hipMemcpyAsync(y + (size_t)r0 * M->nout, M->tin, sizeof(float) * (size_t)rows * M->nout,
+                                  hipMemcpyDeviceToHost, M->c->stream));
+    HIPCHECK(M->c, hipStreamSynchronize(M->c->stream));
+  }
+  return STAB_OK;
+}
+
+int stab_mlp_loss_grad(stab_mlp *M, int n, const float *x, const float *t, float *loss, float *grad) {
+  if (!M || !x || !t) return STAB_E_ARG;
+  int rc;
+  if ((rc = mlp_check_rows(M, n)) != STAB_OK) return rc;
+  HIPCHECK(M->c, hipSetDevice(M->c->device));
+  if ((rc = mlp_reserve(M, n)) != STAB_OK) return rc;
+  HIPCHECK(M->c, hipMemcpyAsync(M->xin, x, sizeof(float) * (size_t)n * M->nin, hipMemcpyHostToDevice, M->c->stream));
+  HIPCHECK(M->c, hipMemcpyAsync(M->tin, t, sizeof(float) * (size_t)n * M->nout, hipMemcpyHostToDevice, M->c->stream));
+  hipLaunchKernelGGL(k_mlp_normalize, dim3(n), dim3(64), 0, M->c->stream, M->nin, M->nin_p, M->nout, M->xin, M->tin,
+                     M->scl, M->act[0], M->Tb);
+  mlp_forward(M, n);
+  hipLaunchKernelGGL(k_mlp_loss, dim3(1), dim3(256), 0, M->c->stream, n, M->nout, M->nout_p, M->Y, M->Tb, M->dY,
+                     M->loss, -1);
+  mlp_backward(M, n);
+  HIPCHECK(M->c, hipGetLastError());
+  if (loss) HIPCHECK(M->c, hipMemcpyAsync(loss, M->loss, sizeof(float), hipMemcpyDeviceToHost, M->c->stream));
+  if (grad) HIPCHECK(M->c, hipMemcpyAsync(grad, M->g, sizeof(float) * M->np, hipMemcpyDeviceToHost, M->c->stream));
+  HIPCHECK(M->c, hipStreamSynchronize(M->c->stream));
+  return STAB_OK;
+}
+
+int stab_mlp_train(stab_mlp *M, int n, const float *x, const float *t, int batch, int steps, float lr, uint64_t seed,
+                   float *loss_hist) {
+  if (!M || !x || !t || steps < 1 || !(lr > 0.f)) return STAB_E_ARG;
+  int rc;
+  if ((rc = mlp_check_rows(M, batch)) != STAB_OK) return rc;
+  if (n < batch) return fail(M->c, STAB_E_ARG, "MLP training set (%d rows) smaller than the batch (%d)", n, batch);
+  HIPCHECK(M->c, hipSetDevice(M->c->device));
+  if ((rc = mlp_reserve(M, batch)) != STAB_OK) return rc;
+  if ((rc = mlp_reserve_data(M, n)) != STAB_OK) return rc;
+  if (loss_hist && steps > M->hist_cap) {
+    mlp_free(M, M->loss);
+    if ((rc = mlp_get(M, sizeof(float) * (size_t)(steps + 1), &M->loss)) != STAB_OK) return rc;
+    M->hist_cap = steps;
+  }
+  hipStream_t s = M->c->stream;
+  HIPCHECK(M->c, hipMemcpyAsync(M->xraw, x, sizeof(float) * (size_t)n * M->nin, hipMemcpyHostToDevice, s));
+  HIPCHECK(M->c, hipMemcpyAsync(M->traw, t, sizeof(float) * (size_t)n * M->nout, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_mlp_normalize, dim3(n), dim3(64), 0, s, M->nin, M->nin_p, M->nout, M->xraw, M->traw, M->scl,
+                     M->Xall, M->Tall);
+  for (int st = 0; st < steps; ++st) {
+    hipLaunchKernelGGL(k_mlp_gather, dim3(batch), dim3(256), 0, s, batch, n, M->nin_p, M->nout, M->Xall, M->Tall,
+                       M->act[0], M->Tb, (unsigned long long)seed, st);
+    mlp_forward(M, batch);
+    hipLaunchKernelGGL(k_mlp_loss, dim3(1), dim3(256), 0, s, batch, M->nout, M->nout_p, M->Y, M->Tb, M->dY, M->loss,
+                       loss_hist ? st : -1);
+    mlp_backward(M, batch);
+    ++M->t;
+    hipLaunchKernelGGL(k_mlp_adam, dim3((M->np + 255) / 256), dim3(256), 0, s, M->np, M->p, M->g, M->m, M->v, lr, 0.9f,
+                       0.999f, 1e-8f, M->t);
+  }
+  HIPCHECK(M->c, hipGetLastError());
+  if (loss_hist)
+    HIPCHECK(M->c, hipMemcpyAsync(loss_hist, M->loss + 1, sizeof(float) * steps, hipMemcpyDeviceToHost, s));
+  HIPCHECK(M->c, hipStreamSynchronize(s));
   return STAB_OK;
 }
 

@@ -1,0 +1,284 @@
+"""Learned shift for the leading-mode solve.
+
+A small MLP (stab_mlp_* in libstab_hip, trained and evaluated by the
+library's own HIP kernels) maps the parameters of a temporal stability point
+of the compressible Blasius family to its leading eigenvalue; the prediction
+is the shift of the residual inverse iteration (stab_solve_leading with
+STAB_LEAD_INVIT), with the plain FP64 Krylov-Schur path as the fallback and as
+the source of every training label.  The reference has no such component: its
+user supplies the guess (getevec -v, thesis/TStest/run.sh:13-17).
+
+Inputs  x = (alpha, beta, log Re, M, Tw/Taw), each mapped to [-1, 1] over the
+        family's ranges.
+Targets t = (omega_r / alpha, omega_i / alpha), the complex phase speed of the
+        leading mode, standardised by the training set's mean and spread.
+
+Mean-flow family: compressible Blasius layers (Sutherland viscosity,
+Te = 288.15 K, Pr = 0.72, gamma = 1.4) on a grid of Mach numbers and wall
+temperatures Tw = (Tw/Taw) Taw; each grid node is one profile slot of the
+solver context, and a point takes the (M, Tw/Taw) of its node.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import dataclasses
+import json
+import math
+import os
+import sys
+import time
+from typing import Sequence
+
+import numpy as np
+
+from . import _lib
+from . import meanflow as mf
+from . import solver as sv
+
+NIN, NOUT = 5, 2
+RANGES = {"alpha": (0.05, 0.4), "beta": (0.0, 0.2), "logRe": (math.log(300.0), math.log(3000.0)),
+          "M": (0.1, 2.5), "Tw": (0.5, 1.5)}
+WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "shift_mlp.npz")
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class ShiftMLP:
+    """Device-resident network of one solver context (stab_mlp_*)."""
+
+    def __init__(self, solver: sv.Solver, width: int = 128, depth: int = 3, seed: int = 1):
+        self.S, self.lib = solver, solver.lib
+        self.width, self.depth = width, depth
+        self.h = C.c_void_p()
+        self.S._check(self.lib.stab_mlp_create(solver.ctx, NIN, width, depth, NOUT, seed, C.byref(self.h)))
+        self.np = self.lib.stab_mlp_nparams(self.h)
+        self.scaling = None
+        in_shift = np.array([0.5 * (a + b) for a, b in RANGES.values()], np.float32)
+        in_scale = np.array([2.0 / (b - a) for a, b in RANGES.values()], np.float32)
+        self.set_scaling(in_shift, in_scale, np.zeros(NOUT, np.float32), np.ones(NOUT, np.float32))
+
+    def close(self):
+        if self.h:
+            self.lib.stab_mlp_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def set_scaling(self, in_shift, in_scale, out_shift, out_scale):
+        arrs = [np.ascontiguousarray(a, dtype=np.float32) for a in (in_shift, in_scale, out_shift, out_scale)]
+        self.S._check(self.lib.stab_mlp_set_scaling(self.h, *[_fp(a) for a in arrs]))
+        self.scaling = arrs
+
+    def get_params(self):
+        p = np.zeros(self.np, np.float32)
+        self.S._check(self.lib.stab_mlp_get_params(self.h, _fp(p)))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.shape == (self.np,)
+        self.S._check(self.lib.stab_mlp_set_params(self.h, _fp(p)))
+
+    def predict(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros((x.shape[0], NOUT), np.float32)
+        self.S._check(self.lib.stab_mlp_predict(self.h, x.shape[0], _fp(x), _fp(y)))
+        return y
+
+    def loss_grad(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        loss = np.zeros(1, np.float32)
+        g = np.zeros(self.np, np.float32)
+        self.S._check(self.lib.stab_mlp_loss_grad(self.h, x.shape[0], _fp(x), _fp(t), _fp(loss), _fp(g)))
+        return float(loss[0]), g
+
+    def train(self, x, t, batch=1024, steps=1000, lr=1e-3, seed=7):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        hist = np.zeros(steps, np.float32)
+        self.S._check(self.lib.stab_mlp_train(self.h, x.shape[0], _fp(x), _fp(t), batch, steps, lr, seed, _fp(hist)))
+        return hist
+
+    def save(self, path, meta=None):
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        np.savez(path, params=self.get_params(), in_shift=self.scaling[0], in_scale=self.scaling[1],
+                 out_shift=self.scaling[2], out_scale=self.scaling[3], width=self.width, depth=self.depth,
+                 meta=json.dumps(meta or {}))
+
+    @classmethod
+    def load(cls, solver: sv.Solver, path: str = WEIGHTS):
+        d = np.load(path, allow_pickle=False)
+        m = cls(solver, int(d["width"]), int(d["depth"]))
+        m.set_scaling(d["in_shift"], d["in_scale"], d["out_shift"], d["out_scale"])
+        m.set_params(d["params"])
+        m.meta = json.loads(str(d["meta"]))
+        return m
+
+
+# ---------------------------------------------------------------------------
+# Mean-flow family and points
+# ---------------------------------------------------------------------------
+@dataclasses.dataclass
+class Family:
+    machs: Sequence[float] = tuple(np.round(np.linspace(0.1, 2.5, 13), 6))
+    tws: Sequence[float] = (0.5, 0.75, 1.0, 1.25, 1.5)
+    nrows: int = 1000
+    Te: float = 288.15
+
+    def nodes(self):
+        return [(float(m), float(w)) for m in self.machs for w in self.tws]
+
+
+def _profile_job(args):
+    m, w, nrows, Te = args
+    return mf.profile(mf.BlasiusSpec(Ma=m, Tw_aw=w, Te=Te, nrows=nrows))
+
+
+def build_profiles(fam: Family, workers: int = 8):
+    """Profile tables of every family node (host solves, in parallel)."""
+    jobs = [(m, w, fam.nrows, fam.Te) for m, w in fam.nodes()]
+    if workers > 1:
+        import multiprocessing as mp
+        with mp.get_context("spawn").Pool(min(workers, len(jobs))) as pool:
+            return pool.map(_profile_job, jobs)
+    return [_profile_job(j) for j in jobs]
+
+
+def load_family(S: sv.Solver, fam: Family, tables):
+    for slot, tab in enumerate(tables):
+        S.set_profile(slot, tab[:, 0], tab[:, 1:6])
+
+
+def sample_points(fam: Family, n: int, seed: int, ny: int = 96, nev: int = 8):
+    """n seeded temporal points over the family; returns (points, features)."""
+    rng = np.random.default_rng(seed)
+    nodes = fam.nodes()
+    node = rng.integers(0, len(nodes), n)
+    al = rng.uniform(*RANGES["alpha"], n)
+    be = rng.uniform(*RANGES["beta"], n)
+    re = np.exp(rng.uniform(*RANGES["logRe"], n))
+    pts, feats = [], np.zeros((n, NIN), np.float32)
+    for k in range(n):
+        M, tw = nodes[node[k]]
+        inp = sv.Inputs(mattyp=1, T0=fam.Te, Te=fam.Te, Ma=M, Re=float(re[k]), Pr=0.72, ny=ny, yi=2.0 / 46.0,
+                        ymax=50.0, itype=1, alpha=complex(al[k], 0.0), beta=complex(be[k], 0.0))
+        pts.append(sv.make_point(inp, complex(0.33 * al[k], 0.0), nev, int(node[k])))
+        feats[k] = (al[k], be[k], math.log(re[k]), M, tw)
+    return pts, feats
+
+
+def labels(S: sv.Solver, pts, chunk: int = 8192):
+    """Leading eigenvalue of every point by the plain FP64 Krylov-Schur path
+    (NaN where no discrete pair converged)."""
+    out = np.full(len(pts), np.nan + 1j * np.nan)
+    for c0 in range(0, len(pts), chunk):
+        r = S.solve_leading(pts[c0:c0 + chunk], _lib.MODE_PLAIN_F64, _lib.LEAD_KRYLOV)
+        ok = r["info"] == _lib.INFO_OK
+        out[c0:c0 + chunk][ok] = r["evals"][ok]
+    return out
+
+
+def targets(feats, omega):
+    a = feats[:, 0].astype(np.float64)
+    return np.stack([omega.real / a, omega.imag / a], 1).astype(np.float32)
+
+
+def shifts(mlp: ShiftMLP, feats):
+    """Predicted leading eigenvalues omega = alpha c of the points."""
+    c = mlp.predict(feats).astype(np.float64)
+    return feats[:, 0] * (c[:, 0] + 1j * c[:, 1])
+
+
+def with_shifts(pts, om):
+    out = []
+    for p, s in zip(pts, om):
+        q = sv.clone_point(p)
+        q.shift.re, q.shift.im = float(s.real), float(s.imag)
+        out.append(q)
+    return out
+
+
+def fit(S: sv.Solver, feats, tgt, width=128, depth=3, seed=1, schedule=((6000, 1e-3), (3000, 3e-4), (3000, 1e-4)),
+        batch=1024):
+    """Train a fresh network on (feats, tgt) with the GPU kernels; returns
+    (network, loss history)."""
+    mlp = ShiftMLP(S, width, depth, seed)
+    mu, sd = tgt.mean(0), tgt.std(0) + 1e-12
+    mlp.set_scaling(mlp.scaling[0], mlp.scaling[1], mu.astype(np.float32), (1.0 / sd).astype(np.float32))
+    hist = []
+    for k, (steps, lr) in enumerate(schedule):
+        hist.append(mlp.train(feats, tgt, batch=batch, steps=steps, lr=lr, seed=seed * 1000 + k))
+    return mlp, np.concatenate(hist)
+
+
+def evaluate(S: sv.Solver, mlp: ShiftMLP, pts, feats, ref, maxit=20):
+    """Leading-mode solves with predicted shifts (FP32 factors, FP64 fallback
+    from the input shift) against reference eigenvalues `ref`."""
+    fb = [complex(p.shift.re, p.shift.im) for p in pts]
+    t0 = time.perf_counter()
+    om = shifts(mlp, feats)
+    t1 = time.perf_counter()
+    r = S.solve_leading(with_shifts(pts, om), _lib.MODE_MIXED, _lib.LEAD_INVIT, maxit=maxit, fallback_shifts=fb)
+    t2 = time.perf_counter()
+    ok = np.isfinite(ref)
+    match = ok & (np.abs(r["evals"] - ref) <= 1e-8 * np.abs(ref))
+    rel = np.abs(om - ref) / np.abs(ref)
+    return dict(n=len(pts), predict_s=t1 - t0, solve_s=t2 - t1, info=np.bincount(r["info"], minlength=4).tolist(),
+                match=int(match.sum()), comparable=int(ok.sum()),
+                shift_rel_err_median=float(np.nanmedian(rel[ok])) if ok.any() else None,
+                shift_rel_err_p95=float(np.nanpercentile(rel[ok], 95)) if ok.any() else None,
+                iters_mean=float(r["iters"][r["info"] == 0].mean()) if (r["info"] == 0).any() else None,
+                stats=r["stats"], result=r)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description="generate labels with the plain path, train the shift MLP on the GPU, "
+                                             "evaluate it on held-out points, save the weights")
+    ap.add_argument("--train-points", type=int, default=200000)
+    ap.add_argument("--eval-points", type=int, default=8192)
+    ap.add_argument("--ny-train", type=int, default=96)
+    ap.add_argument("--ny-eval", type=int, default=128)
+    ap.add_argument("--steps", type=int, default=12000)
+    ap.add_argument("--seed", type=int, default=11)
+    ap.add_argument("--workers", type=int, default=8)
+    ap.add_argument("--out", default=WEIGHTS)
+    ap.add_argument("--summary", default=None)
+    a = ap.parse_args(argv)
+    fam = Family()
+    t0 = time.perf_counter()
+    tables = build_profiles(fam, a.workers)
+    print(f"# {len(tables)} mean flows in {time.perf_counter() - t0:.1f} s", flush=True)
+    S = sv.Solver(0)
+    load_family(S, fam, tables)
+    pts, feats = sample_points(fam, a.train_points, a.seed, a.ny_train)
+    t0 = time.perf_counter()
+    om = labels(S, pts)
+    tl = time.perf_counter() - t0
+    ok = np.isfinite(om)
+    print(f"# labels: {int(ok.sum())}/{len(pts)} points with a leading mode, {tl:.1f} s", flush=True)
+    s1 = int(a.steps * 0.5)
+    sched = ((s1, 1e-3), (int(a.steps * 0.25), 3e-4), (a.steps - s1 - int(a.steps * 0.25), 1e-4))
+    t0 = time.perf_counter()
+    mlp, hist = fit(S, feats[ok], targets(feats[ok], om[ok]), seed=a.seed, schedule=sched)
+    tt = time.perf_counter() - t0
+    print(f"# trained {a.steps} steps in {tt:.1f} s, loss {hist[:10].mean():.3e} -> {hist[-100:].mean():.3e}",
+          flush=True)
+    ept, ef = sample_points(fam, a.eval_points, a.seed + 1, a.ny_eval)
+    ref = labels(S, ept)
+    ev = evaluate(S, mlp, ept, ef, ref)
+    ev.pop("result")
+    print(f"# held-out: {json.dumps(ev)}", flush=True)
+    meta = dict(train_points=a.train_points, labelled=int(ok.sum()), ny_train=a.ny_train, steps=a.steps,
+                seed=a.seed, family=dict(machs=list(fam.machs), tws=list(fam.tws)), loss_final=float(hist[-100:].mean()),
+                heldout=dict((k, v) for k, v in ev.items() if k != "stats"))
+    mlp.save(a.out, meta)
+    if a.summary:
+        with open(a.summary, "w") as f:
+            json.dump(dict(meta=meta, label_seconds=tl, train_seconds=tt, loss_hist=hist[::100].tolist()), f)
+    print(f"# saved {a.out}", flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -36,6 +36,7 @@ class BlasiusSpec:
     Te: float = 288.15           # edge temperature in K for the Sutherland law
     S: float = 110.4             # Sutherland constant in K (input.f90:29-31 uses 110.4)
     Tw_ratio: float | None = None    # None: adiabatic wall; else Tw/Te
+    Tw_aw: float | None = None       # if set: isothermal wall at Tw = Tw_aw * Taw (adiabatic wall temperature)
     eta_max: float = 20.0
     nrows: int = 2000
     tol: float = 1e-12
@@ -84,18 +85,12 @@ def _integrate(spec: BlasiusSpec, F0, gw_or_Q0, dense=False):
     return sol
 
 
-def solve(spec: BlasiusSpec):
-    """Newton shooting on (f''(0), g(0)) [adiabatic] or (f''(0), g'(0)) [isothermal]
-    so that f'(eta_max) = 1 and g(eta_max) = 1.  Returns the dense solution and
-    the wall values."""
-    # Crocco-Busemann starting guess for the wall temperature
-    taw = 1.0 + spec.Pr ** 0.5 * 0.5 * (spec.gamma - 1.0) * spec.Ma ** 2
-    x = np.array([0.4696, taw if spec.Tw_ratio is None else 0.0])
+def _newton(spec: BlasiusSpec, x):
     for _ in range(60):
         sol = _integrate(spec, x[0], x[1])
         r = np.array([sol.y[1, -1] - 1.0, sol.y[3, -1] - 1.0])
         if np.max(np.abs(r)) < 1e-13:
-            break
+            return x
         J = np.empty((2, 2))
         for k in range(2):
             h = 1e-7 * max(1.0, abs(x[k]))
@@ -103,8 +98,30 @@ def solve(spec: BlasiusSpec):
             sp = _integrate(spec, xp[0], xp[1])
             J[:, k] = (np.array([sp.y[1, -1] - 1.0, sp.y[3, -1] - 1.0]) - r) / h
         x = x - np.linalg.solve(J, r)
-    else:
-        raise RuntimeError("shooting did not converge")
+    raise RuntimeError("shooting did not converge")
+
+
+def solve(spec: BlasiusSpec):
+    """Newton shooting on (f''(0), g(0)) [adiabatic] or (f''(0), g'(0)) [isothermal]
+    so that f'(eta_max) = 1 and g(eta_max) = 1.  Returns the dense solution and
+    the wall values.  With Tw_aw set, the adiabatic problem is solved first and
+    the wall temperature is stepped from Taw to Tw_aw * Taw, each isothermal
+    Newton solve starting from the previous one."""
+    # Crocco-Busemann starting guess for the wall temperature
+    taw = 1.0 + spec.Pr ** 0.5 * 0.5 * (spec.gamma - 1.0) * spec.Ma ** 2
+    if spec.Tw_aw is not None:
+        ad = dataclasses.replace(spec, Tw_aw=None, Tw_ratio=None)
+        xa = _newton(ad, np.array([0.4696, taw]))
+        g_aw = xa[1]
+        x = np.array([xa[0], 0.0])
+        steps = max(1, int(math.ceil(abs(spec.Tw_aw - 1.0) / 0.1)))
+        for k in range(1, steps + 1):
+            gw = g_aw * (1.0 + (spec.Tw_aw - 1.0) * k / steps)
+            x = _newton(dataclasses.replace(spec, Tw_aw=None, Tw_ratio=gw), x)
+        iso = dataclasses.replace(spec, Tw_aw=None, Tw_ratio=g_aw * spec.Tw_aw)
+        return _integrate(iso, x[0], x[1], dense=True), x
+    x = np.array([0.4696, taw if spec.Tw_ratio is None else 0.0])
+    x = _newton(spec, x)
     return _integrate(spec, x[0], x[1], dense=True), x
 
 
