@@ -1,4 +1,4 @@
-"""Throughput benchmark of the batched Chebyshev stability solve on one MI355X.
+"""Throughput benchmark of the batched leading-mode stability solve on one MI355X.
 
 Workload (one step = one pass of the hot path over the batch): a seeded sweep
 of temporal stability points of a compressible Blasius boundary layer at
@@ -6,16 +6,26 @@ M = 0.5 (Pr = 0.72, gamma = 1.4, Sutherland viscosity with Te = 288.15 K,
 adiabatic wall): alpha ~ U[0.05, 0.4], beta ~ U[0, 0.2], Re ~ U[300, 3000],
 N ~ U{64, 96, 128} (pencils 320^2 .. 640^2 complex), Streett mapping
 ymax = 50 with half-point y = 2.  Each point carries an input-file shift
-sigma = 0.33 alpha (a phase-speed guess, as a getevec -v guess) and asks for
-the 8 eigenpairs nearest it.  A mode counts as converged when its FP64
-relative residual ||(A - w B) x|| / ((||A|| + |w| ||B||) ||x||) < 1e-10.
+sigma = 0.33 alpha (a phase-speed guess, as a getevec -v guess).
 
-value = converged modes per second of the faster GPU path on this workload.
-Both paths are timed on the same batch: ``plain`` (FP64 LU + FP64 shift-invert
-Krylov-Schur) and ``mixed`` (equilibrated FP32 LU, FP64 iterative refinement
-inside every operator application, FP64 fallback per point).  The CPU baseline
-times the oracle (ZGESV + ZGEEV, the reference's method) on a bounded sample
-and checks the GPU eigenvalues against ZGGEV on the same points.
+The quantity is the leading eigenmode of every point: its least stable
+discrete mode (decaying in the freestream, off the c = 1 continuum).
+
+  plain   FP64 LU + FP64 shift-invert Krylov-Schur for the 8 pairs nearest
+          sigma, then the leading discrete pair among them.  It is the
+          reference answer of every point.
+  chosen  The shift MLP (trained on plain-path solves by the library's own
+          GPU kernels, stab_amd/data/shift_mlp.npz) predicts the leading
+          eigenvalue; FP32 LU on MFMA at that shift; FP64 residual inverse
+          iteration; FP64 Krylov-Schur from sigma for points whose residual
+          stays above 1e-13 or that land on a non-discrete mode.
+
+value = chosen-path leading eigenmodes per second, counting a point when its
+eigenvalue equals the plain path's leading eigenvalue to 1e-8 (relative).
+vs_plain = value / plain-path leading eigenmodes per second.  The CPU
+baseline times the oracle's restatement of the reference method (ZGESV +
+ZGEEV, the whole spectrum) on a bounded sample and checks the chosen path's
+eigenvalues against ZGGEV on the same points.
 """
 from __future__ import annotations
 
@@ -35,6 +45,7 @@ sys.path.insert(0, ROOT)
 PEAK_HBM_GBS = 8000.0
 PEAK_FP32_MFMA_TFLOPS = 157.3
 PEAK_FP64_MFMA_TFLOPS = 78.6
+LU32_KERNEL = "k_lu<float, 16, 256, 3, 16>"
 
 
 def make_workload(npts: int, seed: int):
@@ -58,6 +69,13 @@ def make_points(al, be, re, ny, nev=8):
     return pts
 
 
+def features(al, be, re):
+    """Shift-MLP inputs of the sweep points: (alpha, beta, log Re, M, Tw/Taw),
+    the adiabatic wall being Tw/Taw = 1."""
+    n = len(al)
+    return np.stack([al, be, np.log(re), np.full(n, 0.5), np.full(n, 1.0)], 1).astype(np.float32)
+
+
 def load_traffic(kernel: str, key: str):
     """HBM bytes per launch of `kernel` from the committed counter summary
     whose workload-key line equals `key` (profiles/*_hbm_traffic.md), or None."""
@@ -68,7 +86,7 @@ def load_traffic(kernel: str, key: str):
             continue
         for line in text.splitlines():
             cells = [c.strip() for c in line.strip().strip("|").split("|")]
-            if len(cells) == 5 and cells[0].startswith(kernel + "<") or (len(cells) == 5 and cells[0] == kernel):
+            if len(cells) == 5 and cells[0] == kernel:
                 return float(cells[4]) * 1e9
     return None
 
@@ -89,28 +107,26 @@ def cpu_worker(args):
     dt = time.perf_counter() - t0
     wq, _ = so.qz(A0, B0, right=False)
     wq = wq[np.isfinite(wq)]
-    return dt, w, wq
+    return dt, wq
 
 
-def cpu_baseline(tab, al, be, re, ny, gpu_evals, nev, sample, workers):
+def cpu_baseline(tab, al, be, re, ny, chosen_evals, sample, workers):
     import multiprocessing as mp
     idx = list(range(sample))
     jobs = [(tab, al[i], be[i], re[i], ny[i]) for i in idx]
     t0 = time.perf_counter()
-    ctx = mp.get_context("spawn")
-    with ctx.Pool(workers) as pool:
+    with mp.get_context("spawn").Pool(workers) as pool:
         out = pool.map(cpu_worker, jobs)
     wall = time.perf_counter() - t0
-    # QZ check of the GPU eigenvalues on the sampled points
     match = total = 0
-    for (dt, w, wq), i in zip(out, idx):
-        for k in range(nev):
-            ev = gpu_evals[i, k]
-            d = np.min(np.abs(wq - ev))
-            total += 1
-            match += int(d <= 1e-8 * abs(ev))
+    for (dt, wq), i in zip(out, idx):
+        ev = chosen_evals[i]
+        if not np.isfinite(ev) or ev == 0:
+            continue
+        total += 1
+        match += int(np.min(np.abs(wq - ev)) <= 1e-8 * abs(ev))
     per_point = sum(o[0] for o in out) / len(out)
-    return dict(value=nev * len(out) / wall, per_point_s=per_point, wall_s=wall, match=match, total=total)
+    return dict(value=len(out) / wall, per_point_s=per_point, wall_s=wall, match=match, total=total)
 
 
 def main():
@@ -121,9 +137,10 @@ def main():
     ap.add_argument("--points", type=int, default=4096)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--nev", type=int, default=8)
+    ap.add_argument("--maxit", type=int, default=20)
     ap.add_argument("--cpu-sample", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--only", choices=["plain", "chosen"], default=None, help="time one path only (profiling)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="run one device-to-device copy of MIB MiB first (reference kernel for HBM byte counters)")
     args = ap.parse_args()
@@ -133,12 +150,16 @@ def main():
         sys.exit(2)
 
     from stab_amd import _lib
+    from stab_amd import learned
     from stab_amd import solver as sv
 
     tab, al, be, re, ny = make_workload(args.points, args.seed)
     pts = make_points(al, be, re, ny, args.nev)
+    feats = features(al, be, re)
+    sigma = np.array([complex(p.shift.re, p.shift.im) for p in pts])
     S = sv.Solver(0)
     S.set_profile(0, tab[:, 0], tab[:, 1:6])
+    mlp = learned.ShiftMLP.load(S)
     copy_gbs = None
     if args.copy_calibration > 0:
         nbytes = args.copy_calibration << 20
@@ -149,90 +170,120 @@ def main():
         ms = S.copy_probe(1 << 30, 10)
         copy_gbs = 2.0 * (1 << 30) / (ms * 1e-3) / 1e9
 
-    def run(mode, steps, warm):
+    def plain_step():
+        return S.solve_leading(pts, _lib.MODE_PLAIN_F64, _lib.LEAD_KRYLOV)
+
+    def chosen_step():
+        om = learned.shifts(mlp, feats)
+        return S.solve_leading(pts, _lib.MODE_MIXED, _lib.LEAD_INVIT, maxit=args.maxit, fallback_shifts=sigma,
+                               shifts=om)
+
+    def timed(fn, steps, warm):
         for _ in range(warm):
-            S.solve(pts, mode, want_vectors=False)
+            fn()
         times, stats, res = [], [], None
         for _ in range(steps):
             t0 = time.perf_counter()
-            res = S.solve(pts, mode, want_vectors=False)
+            res = fn()
             times.append(time.perf_counter() - t0)
-            stats.append(res.stats)
+            stats.append(res["stats"])
         return res, times, stats
 
-    def summarize(res, times, stats):
-        ok = (res.resid[:, :args.nev] < 1e-10) & (res.info[:, None] == 0)
-        nconv = int(ok.sum())
-        t = float(np.mean(times))
-        ms_arn = float(np.mean([s["ms_arnoldi"] for s in stats]))
-        sbytes = float(np.mean([s["solve_bytes"] for s in stats]))
-        ms_lu = float(np.mean([s["ms_lu"] for s in stats]))
-        return dict(value=nconv / t, ms_per_step=t * 1e3, converged=nconv, ms_lu=ms_lu,
-                    ms_eig=float(np.mean([s["ms_eig"] for s in stats])), ms_arnoldi=ms_arn,
-                    arnoldi_gbs=sbytes / (ms_arn * 1e-3) / 1e9 if ms_arn > 0 else None,
-                    lu_tflops=stats[-1]["lu_flops"] / (ms_lu * 1e-3) / 1e12,
-                    ks_cycles_max=stats[-1]["restarts_max"], nfallback=stats[-1]["nfallback"],
-                    op_applies=stats[-1]["op_applies"])
+    def mean_stat(stats, k):
+        return float(np.mean([s[k] for s in stats]))
 
-    # production path for the 8-mode sweep: FP64 LU + FP64 shift-invert Krylov-Schur
-    print(f"# bench: {args.points} points, plain FP64 path", file=sys.stderr, flush=True)
-    res, times, stats = run(_lib.MODE_PLAIN_F64, args.steps, args.warmup)
-    plain = summarize(res, times, stats)
-    print(f"# plain: {plain}", file=sys.stderr, flush=True)
-
-    mixed = None
-    if not args.no_mixed:
-        print("# bench: FP32-LU + FP64-refinement path (1 step)", file=sys.stderr, flush=True)
-        mres, mtimes, mstats = run(_lib.MODE_MIXED, 1, 0)
-        mixed = summarize(mres, mtimes, mstats)
-        print(f"# mixed: {mixed}", file=sys.stderr, flush=True)
-
-    best = plain if (mixed is None or plain["value"] >= mixed["value"]) else mixed
-    best_name = "plain FP64" if best is plain else "FP32 LU + FP64 refinement"
-
-    cpu = None
-    if not args.no_cpu:
-        print("# bench: CPU baseline sample", file=sys.stderr, flush=True)
-        workers = max(1, min(16, os.cpu_count() or 1, args.cpu_sample))
-        cb = cpu_baseline(tab, al, be, re, ny, res.evals, args.nev, args.cpu_sample, workers)
-        cpu = {"value": cb["value"], "unit": "leading eigenmodes/s", "cores": workers, "kind": "port",
-               "sample": f"first {args.cpu_sample} points of the same seeded sweep, reference method "
-                         f"(ZGESV+ZGEEV full spectrum, 8 modes counted per point), one point per process, "
-                         f"OPENBLAS_NUM_THREADS=1; {cb['per_point_s']:.2f} s per point on one core",
-               "qz_match": f"{cb['match']}/{cb['total']} GPU eigenvalues within 1e-8 of a ZGGEV eigenvalue"}
+    plain = chosen = None
+    if args.only in (None, "plain"):
+        print(f"# bench: {args.points} points, plain FP64 Krylov-Schur leading mode", file=sys.stderr, flush=True)
+        pres, ptimes, pstats = timed(plain_step, args.steps, args.warmup)
+        pfound = pres["info"] == _lib.INFO_OK
+        tp = float(np.mean(ptimes))
+        plain = dict(value=int(pfound.sum()) / tp, ms_per_step=tp * 1e3, found=int(pfound.sum()),
+                     ms_lu=mean_stat(pstats, "ms_lu"), ms_lu_kernel=mean_stat(pstats, "ms_lu_kernel"),
+                     ms_eig=mean_stat(pstats, "ms_eig"), ms_arnoldi=mean_stat(pstats, "ms_arnoldi"),
+                     lu_tflops=pstats[-1]["lu_flops"] / (mean_stat(pstats, "ms_lu_kernel") * 1e-3) / 1e12,
+                     arnoldi_gbs=mean_stat(pstats, "solve_bytes") / (mean_stat(pstats, "ms_arnoldi") * 1e-3) / 1e9,
+                     ks_cycles_max=pstats[-1]["restarts_max"], op_applies=pstats[-1]["op_applies"])
+        print(f"# plain: {plain}", file=sys.stderr, flush=True)
+    if args.only in (None, "chosen"):
+        print("# bench: learned shift + FP32 LU + FP64 residual inverse iteration", file=sys.stderr, flush=True)
+        cres, ctimes, cstats = timed(chosen_step, args.steps, args.warmup)
+        tc = float(np.mean(ctimes))
+        delivered = np.isin(cres["info"], [_lib.INFO_OK, _lib.INFO_FALLBACK])
+        chosen = dict(ms_per_step=tc * 1e3, delivered=int(delivered.sum()),
+                      direct=int((cres["info"] == _lib.INFO_OK).sum()),
+                      fallback=int((cres["info"] == _lib.INFO_FALLBACK).sum()),
+                      noconv=int((cres["info"] == _lib.INFO_NOCONV).sum()),
+                      solves_per_point=float(cres["iters"][cres["info"] == _lib.INFO_OK].mean()),
+                      ms_lu=mean_stat(cstats, "ms_lu"), ms_lu_kernel=mean_stat(cstats, "ms_lu_kernel"),
+                      ms_prep=mean_stat(cstats, "ms_prep"), ms_assemble=mean_stat(cstats, "ms_assemble"),
+                      ms_eig=mean_stat(cstats, "ms_eig"), lu_flops=cstats[-1]["lu_flops"])
+        chosen["lu_tflops"] = chosen["lu_flops"] / (chosen["ms_lu_kernel"] * 1e-3) / 1e12
+        print(f"# chosen: {chosen}", file=sys.stderr, flush=True)
 
     out = {
-        "metric": "leading eigenmodes/sec at |dw|<=1e-8 (8 eigenpairs nearest the input shift, FP64 residual < 1e-10)",
-        "value": best["value"],
+        "metric": "leading eigenmodes/sec at |dw|<=1e-8 (least stable discrete mode per point; learned shift + "
+                  "FP32 MFMA LU + FP64 residual inverse iteration, against the plain FP64 Krylov-Schur path)",
         "unit": "leading eigenmodes/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": best["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c128 (f64)" if best is plain else "c64 LU / f64 refinement",
-        "data": "synthetic (seeded parameter sweep over a computed compressible Blasius profile)",
-        "config": {"workload": f"temporal sweep, compressible Blasius M=0.5, {args.points} points, "
-                               "N in {64,96,128}, 8 modes nearest sigma=0.33 alpha",
-                   "path": best_name, "points": args.points, "nev": args.nev, "seed": args.seed,
-                   "converged_modes": best["converged"], "total_modes": int(args.points * args.nev)},
-        "plain": plain,
-        "mixed": mixed,
-        "vs_plain": best["value"] / plain["value"],
-        "roofline": {"bound": "hbm", "kernel": "k_arnoldi (shift-invert Krylov expansion: blocked triangular "
-                                               "solves + CGS2)",
-                     "achieved": plain["arnoldi_gbs"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": (plain["arnoldi_gbs"] or 0.0) / PEAK_HBM_GBS,
-                     "traffic": load_traffic("k_arnoldi", f"temporal-sweep points={args.points} seed={args.seed} "
-                                                          f"nev={args.nev} mode=plain"),
-                     "algorithmic_bytes": "16 npad^2 per operator application (one pass over the FP64 LU factors)",
-                     "measured_copy_gbs": copy_gbs},
-        "lu": {"kernel": "k_lu (batched complex LU, FP64)", "achieved_tflops": plain["lu_tflops"],
-               "peak_tflops": PEAK_FP64_MFMA_TFLOPS, "frac": plain["lu_tflops"] / PEAK_FP64_MFMA_TFLOPS},
-        "cpu_baseline": cpu,
+        "data": "synthetic (seeded parameter sweep over a computed compressible Blasius profile; shift network "
+                "trained on this library's own plain-path solves)",
     }
+    if plain is not None and chosen is not None:
+        match = delivered & pfound & (np.abs(cres["evals"] - pres["evals"]) <= 1e-8 * np.abs(pres["evals"]))
+        chosen["matches_plain"] = int(match.sum())
+        chosen["value"] = int(match.sum()) / (chosen["ms_per_step"] * 1e-3)
+        out.update({
+            "value": chosen["value"],
+            "ms_per_step": chosen["ms_per_step"],
+            "dtype": "c64 LU (FP32 MFMA) + f64 residual inverse iteration; bf16 MLP matmuls",
+            "config": {"workload": f"temporal sweep, compressible Blasius M=0.5, {args.points} points, "
+                                   "N in {64,96,128}, leading discrete mode per point",
+                       "path": "learned shift + FP32 MFMA LU + FP64 residual inverse iteration (FP64 Krylov-Schur "
+                               "fallback)",
+                       "points": args.points, "seed": args.seed, "nev_plain": args.nev, "maxit": args.maxit,
+                       "matched_points": int(match.sum()), "plain_found": int(pfound.sum())},
+            "vs_plain": chosen["value"] / plain["value"],
+            "plain": plain,
+            "chosen": chosen,
+        })
+        lu_ms = chosen["ms_lu_kernel"]
+        achieved = chosen["lu_flops"] / (lu_ms * 1e-3) / 1e12
+        out["roofline"] = {
+            "bound": "mfma", "kernel": LU32_KERNEL + " (batched complex LU, FP32 matrix cores, one launch per step)",
+            "achieved": achieved, "peak": PEAK_FP32_MFMA_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP32_MFMA_TFLOPS,
+            "traffic": load_traffic(LU32_KERNEL, f"temporal-sweep points={args.points} seed={args.seed} "
+                                                 f"path=chosen"),
+            "algorithmic_flops": "8/3 n^3 real flops per n x n complex LU (n = 5N), summed over the batch",
+            "measured_copy_gbs": copy_gbs,
+        }
+        out["secondary"] = {
+            "plain_lu_fp64_tflops": plain["lu_tflops"], "plain_lu_fp64_frac": plain["lu_tflops"] / PEAK_FP64_MFMA_TFLOPS,
+            "plain_arnoldi_gbs": plain["arnoldi_gbs"], "plain_arnoldi_frac": plain["arnoldi_gbs"] / PEAK_HBM_GBS,
+        }
+        if not args.no_cpu:
+            print("# bench: CPU baseline sample", file=sys.stderr, flush=True)
+            workers = max(1, min(16, os.cpu_count() or 1, args.cpu_sample))
+            ev = np.where(delivered, cres["evals"], np.nan)
+            cb = cpu_baseline(tab, al, be, re, ny, ev, args.cpu_sample, workers)
+            out["cpu_baseline"] = {
+                "value": cb["value"], "unit": "leading eigenmodes/s", "cores": workers, "kind": "port",
+                "sample": f"first {args.cpu_sample} points of the same seeded sweep, reference method (ZGESV+ZGEEV, "
+                          f"whole spectrum, one leading mode per point), one point per process, "
+                          f"OPENBLAS_NUM_THREADS=1; {cb['per_point_s']:.2f} s per point on one core",
+                "qz_match": f"{cb['match']}/{cb['total']} chosen-path leading eigenvalues within 1e-8 of a ZGGEV "
+                            f"eigenvalue"}
+        else:
+            out["cpu_baseline"] = None
+    else:
+        one = plain or chosen
+        out.update({"value": None, "ms_per_step": one["ms_per_step"], "partial": plain or chosen})
     print(json.dumps(out))
 
 

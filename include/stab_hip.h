@@ -105,7 +105,8 @@ typedef struct {
 
 typedef struct stab_ctx stab_ctx;
 
-/* Solver statistics of the last stab_solve call (device-timed, ms). */
+/* Solver statistics of the last stab_solve / stab_solve_leading call
+ * (device-timed, ms). */
 typedef struct {
   double ms_prep, ms_assemble, ms_lu, ms_eig, ms_total;
   int64_t lu_flops;       /* algorithmic (8/3) n^3 per point, summed */
@@ -116,6 +117,7 @@ typedef struct {
   double ms_arnoldi;      /* device time of the Arnoldi-expansion kernel launches */
   int64_t op_applies;     /* shift-invert operator applications, summed over points */
   double solve_bytes;     /* algorithmic factor bytes read by those applications */
+  double ms_lu_kernel;    /* device time of the LU kernel alone (ms_lu adds the block inverses) */
 } stab_stats;
 
 /* Context on HIP device `device`.  New in this library (the reference keeps

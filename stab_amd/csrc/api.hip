@@ -282,6 +282,7 @@ int launch_lu(stab_ctx *c, int npts, Batch &B) {
   // at most three panel rows per thread
   int rc = (B.npadmax <= 768) ? launch_lu_nt<T, NB, 256>(c, npts, B) : launch_lu_nt<T, NB, 512>(c, npts, B);
   if (rc != STAB_OK) return rc;
+  HIPCHECK(c, hipEventRecord(c->ev[7], c->stream));   // end of k_lu (its start is ev[2] in the solve paths)
   hipLaunchKernelGGL((k_blkinv<T, SOLVE_BS>), dim3((B.npadmax + SOLVE_BS - 1) / SOLVE_BS, npts), dim3(SOLVE_BS), 0,
                      c->stream, B.pts, (const cx<T> *)B.mats, (cx<T> *)B.blks);
   HIPCHECK(c, hipGetLastError());
@@ -698,7 +699,7 @@ size_t stab_evec_bytes(int npts, const stab_point *pts) {
 }
 
 struct RunInfo {
-  double ms_prep = 0, ms_assemble = 0, ms_lu = 0, ms_eig = 0, ms_total = 0, ms_arnoldi = 0;
+  double ms_prep = 0, ms_assemble = 0, ms_lu = 0, ms_eig = 0, ms_total = 0, ms_arnoldi = 0, ms_lu_kernel = 0;
   int cycles = 0, nconv = 0;
 };
 
@@ -748,6 +749,7 @@ static int solve_run(stab_ctx *c, int npts, const stab_point *pts, int mode, Bat
   ri.ms_prep = elapsed(c->ev[0], c->ev[1]);
   ri.ms_assemble = elapsed(c->ev[1], c->ev[2]);
   ri.ms_lu = elapsed(c->ev[2], c->ev[3]);
+  ri.ms_lu_kernel = elapsed(c->ev[2], c->ev[7]);
   ri.ms_eig = elapsed(c->ev[3], c->ev[4]);
   ri.ms_total = elapsed(c->ev[0], c->ev[4]);
   return STAB_OK;
@@ -759,6 +761,7 @@ static void fill_stats(stab_stats *stats, int npts, const stab_point *pts, const
   stats->ms_prep = ri.ms_prep;
   stats->ms_assemble = ri.ms_assemble;
   stats->ms_lu = ri.ms_lu;
+  stats->ms_lu_kernel = ri.ms_lu_kernel;
   stats->ms_eig = ri.ms_eig;
   stats->ms_total = ri.ms_total;
   for (int i = 0; i < npts; ++i) {
@@ -1034,6 +1037,7 @@ static int lead_invit(stab_ctx *c, int npts, const stab_point *pts, int mode, in
     ri.ms_prep = elapsed(c->ev[0], c->ev[1]);
     ri.ms_assemble = elapsed(c->ev[1], c->ev[2]);
     ri.ms_lu = elapsed(c->ev[2], c->ev[3]);
+    ri.ms_lu_kernel = elapsed(c->ev[2], c->ev[7]);
     ri.ms_eig = elapsed(c->ev[3], c->ev[4]);
     ri.ms_total = elapsed(c->ev[0], c->ev[4]);
     ri.ms_arnoldi = ri.ms_eig;

@@ -220,7 +220,7 @@ def evaluate(S: sv.Solver, mlp: ShiftMLP, pts, feats, ref, maxit=20):
     t0 = time.perf_counter()
     om = shifts(mlp, feats)
     t1 = time.perf_counter()
-    r = S.solve_leading(with_shifts(pts, om), _lib.MODE_MIXED, _lib.LEAD_INVIT, maxit=maxit, fallback_shifts=fb)
+    r = S.solve_leading(pts, _lib.MODE_MIXED, _lib.LEAD_INVIT, maxit=maxit, fallback_shifts=fb, shifts=om)
     t2 = time.perf_counter()
     ok = np.isfinite(ref)
     match = ok & (np.abs(r["evals"] - ref) <= 1e-8 * np.abs(ref))

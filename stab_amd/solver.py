@@ -231,10 +231,17 @@ class Solver:
 
     def solve_leading(self, points: Sequence[Point], mode: int = _lib.MODE_PLAIN_F64,
                       method: int = _lib.LEAD_KRYLOV, maxit: int = 30, fallback_shifts=None,
-                      want_vectors: bool = False):
-        """Leading (least stable discrete) eigenmode per point (stab_solve_leading)."""
+                      want_vectors: bool = False, shifts=None):
+        """Leading (least stable discrete) eigenmode per point (stab_solve_leading).
+        `shifts` (complex, one per point) replaces the points' shifts."""
         npts = len(points)
         arr = (Point * npts)(*points)
+        if shifts is not None:
+            sh = np.asarray(shifts, dtype=np.complex128)
+            assert sh.shape == (npts,)
+            view = np.ctypeslib.as_array(arr)
+            view["shift"]["re"] = sh.real
+            view["shift"]["im"] = sh.imag
         ev = np.zeros(npts, dtype=np.complex128)
         rs = np.zeros(npts)
         ff = np.zeros(npts)
