@@ -69,25 +69,40 @@ __device__ void lu_solve(const PointDev &P, const cx<T> *__restrict__ A, const i
       if (tq == 0) tmp[ti] = s;
     }
     __syncthreads();
-    zd acc[RS];
-#pragma unroll
-    for (int j = 0; j < RS; ++j) {
-      const int r = k0 + bs + t + j * NT;
-      acc[j] = (r < n) ? y[r] : czero<double>();
-    }
-    for (int c = 0; c < bs; ++c) {
-      const zd tc = tmp[c];
-      const cx<T> *col = A + (size_t)(k0 + c) * lda;
+    if (bs == BS) {
+      // rows below the block, one at a time, eight column loads in flight
+#pragma unroll 1
+      for (int j = 0; j < RS; ++j) {
+        const int r = k0 + BS + t + j * NT;
+        if (r < n) {
+          zd a = y[r];
+          const cx<T> *col = A + (size_t)k0 * lda + r;
+#pragma unroll 4
+          for (int c = 0; c < BS; ++c) a = cfms(a, cvt<double>(col[(size_t)c * lda]), tmp[c]);
+          y[r] = a;
+        }
+      }
+    } else {
+      zd acc[RS];
 #pragma unroll
       for (int j = 0; j < RS; ++j) {
         const int r = k0 + bs + t + j * NT;
-        if (r < n) acc[j] = cfms(acc[j], cvt<double>(col[r]), tc);
+        acc[j] = (r < n) ? y[r] : czero<double>();
       }
-    }
+      for (int c = 0; c < bs; ++c) {
+        const zd tc = tmp[c];
+        const cx<T> *col = A + (size_t)(k0 + c) * lda;
 #pragma unroll
-    for (int j = 0; j < RS; ++j) {
-      const int r = k0 + bs + t + j * NT;
-      if (r < n) y[r] = acc[j];
+        for (int j = 0; j < RS; ++j) {
+          const int r = k0 + bs + t + j * NT;
+          if (r < n) acc[j] = cfms(acc[j], cvt<double>(col[r]), tc);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < RS; ++j) {
+        const int r = k0 + bs + t + j * NT;
+        if (r < n) y[r] = acc[j];
+      }
     }
     if (t < bs) y[k0 + t] = tmp[t];
     __syncthreads();
@@ -103,25 +118,39 @@ __device__ void lu_solve(const PointDev &P, const cx<T> *__restrict__ A, const i
       if (tq == 0) tmp[ti] = s;
     }
     __syncthreads();
-    zd acc[RS];
-#pragma unroll
-    for (int j = 0; j < RS; ++j) {
-      const int r = t + j * NT;
-      acc[j] = (r < k0) ? y[r] : czero<double>();
-    }
-    for (int c = 0; c < bs; ++c) {
-      const zd tc = tmp[c];
-      const cx<T> *col = A + (size_t)(k0 + c) * lda;
+    if (bs == BS) {
+#pragma unroll 1
+      for (int j = 0; j < RS; ++j) {
+        const int r = t + j * NT;
+        if (r < k0) {
+          zd a = y[r];
+          const cx<T> *col = A + (size_t)k0 * lda + r;
+#pragma unroll 4
+          for (int c = 0; c < BS; ++c) a = cfms(a, cvt<double>(col[(size_t)c * lda]), tmp[c]);
+          y[r] = a;
+        }
+      }
+    } else {
+      zd acc[RS];
 #pragma unroll
       for (int j = 0; j < RS; ++j) {
         const int r = t + j * NT;
-        if (r < k0) acc[j] = cfms(acc[j], cvt<double>(col[r]), tc);
+        acc[j] = (r < k0) ? y[r] : czero<double>();
       }
-    }
+      for (int c = 0; c < bs; ++c) {
+        const zd tc = tmp[c];
+        const cx<T> *col = A + (size_t)(k0 + c) * lda;
 #pragma unroll
-    for (int j = 0; j < RS; ++j) {
-      const int r = t + j * NT;
-      if (r < k0) y[r] = acc[j];
+        for (int j = 0; j < RS; ++j) {
+          const int r = t + j * NT;
+          if (r < k0) acc[j] = cfms(acc[j], cvt<double>(col[r]), tc);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < RS; ++j) {
+        const int r = t + j * NT;
+        if (r < k0) y[r] = acc[j];
+      }
     }
     if (t < bs) y[k0 + t] = tmp[t];
     __syncthreads();
