@@ -283,7 +283,7 @@ int launch_lu(stab_ctx *c, int npts, Batch &B) {
   int rc = (B.npadmax <= 768) ? launch_lu_nt<T, NB, 256>(c, npts, B) : launch_lu_nt<T, NB, 512>(c, npts, B);
   if (rc != STAB_OK) return rc;
   HIPCHECK(c, hipEventRecord(c->ev[7], c->stream));   // end of k_lu (its start is ev[2] in the solve paths)
-  hipLaunchKernelGGL((k_blkinv<T, SOLVE_BS>), dim3((B.npadmax + SOLVE_BS - 1) / SOLVE_BS, npts), dim3(SOLVE_BS), 0,
+  hipLaunchKernelGGL((k_blkinv<T, SOLVE_BS>), dim3((B.npadmax + SOLVE_BS - 1) / SOLVE_BS, npts), dim3(4 * SOLVE_BS), 0,
                      c->stream, B.pts, (const cx<T> *)B.mats, (cx<T> *)B.blks);
   HIPCHECK(c, hipGetLastError());
   return STAB_OK;
@@ -730,13 +730,13 @@ static int solve_run(stab_ctx *c, int npts, const stab_point *pts, int mode, Bat
   HIPCHECK(c, hipEventRecord(c->ev[0], c->stream));
   if ((rc = run_prep(c, npts, B, f32)) != STAB_OK) return rc;
   HIPCHECK(c, hipEventRecord(c->ev[1], c->stream));
-  const int cb = (B.npadmax + 7) / 8;
+  const int rb = (B.npadmax + 255) / 256;
   if (f32)
-    hipLaunchKernelGGL((k_assemble<float, 8>), dim3(cb, npts), dim3(256), 0, c->stream, B.pts, B.coef, B.ops, B.scales,
-                       (zf *)B.mats, (int)FORM_SHIFTED, 1);
+    hipLaunchKernelGGL((k_assemble_rows<float>), dim3(rb, npts), dim3(256), 0, c->stream, B.pts, B.coef, B.ops,
+                       B.scales, (zf *)B.mats, 1);
   else
-    hipLaunchKernelGGL((k_assemble<double, 8>), dim3(cb, npts), dim3(256), 0, c->stream, B.pts, B.coef, B.ops,
-                       B.scales, (zd *)B.mats, (int)FORM_SHIFTED, 0);
+    hipLaunchKernelGGL((k_assemble_rows<double>), dim3(rb, npts), dim3(256), 0, c->stream, B.pts, B.coef, B.ops,
+                       B.scales, (zd *)B.mats, 0);
   HIPCHECK(c, hipGetLastError());
   HIPCHECK(c, hipEventRecord(c->ev[2], c->stream));
   if ((rc = f32 ? launch_lu<float, 16>(c, npts, B) : launch_lu<double, 8>(c, npts, B)) != STAB_OK) return rc;
@@ -982,13 +982,13 @@ static int lead_invit(stab_ctx *c, int npts, const stab_point *pts, int mode, in
   HIPCHECK(c, hipEventRecord(c->ev[0], c->stream));
   if ((rc = run_prep(c, npts, B, f32)) != STAB_OK) return rc;
   HIPCHECK(c, hipEventRecord(c->ev[1], c->stream));
-  const int cb = (B.npadmax + 7) / 8;
+  const int rb = (B.npadmax + 255) / 256;
   if (f32)
-    hipLaunchKernelGGL((k_assemble<float, 8>), dim3(cb, npts), dim3(256), 0, c->stream, B.pts, B.coef, B.ops, B.scales,
-                       (zf *)B.mats, (int)FORM_SHIFTED, 1);
+    hipLaunchKernelGGL((k_assemble_rows<float>), dim3(rb, npts), dim3(256), 0, c->stream, B.pts, B.coef, B.ops,
+                       B.scales, (zf *)B.mats, 1);
   else
-    hipLaunchKernelGGL((k_assemble<double, 8>), dim3(cb, npts), dim3(256), 0, c->stream, B.pts, B.coef, B.ops,
-                       B.scales, (zd *)B.mats, (int)FORM_SHIFTED, 0);
+    hipLaunchKernelGGL((k_assemble_rows<double>), dim3(rb, npts), dim3(256), 0, c->stream, B.pts, B.coef, B.ops,
+                       B.scales, (zd *)B.mats, 0);
   HIPCHECK(c, hipGetLastError());
   HIPCHECK(c, hipEventRecord(c->ev[2], c->stream));
   if ((rc = f32 ? launch_lu<float, 16>(c, npts, B) : launch_lu<double, 8>(c, npts, B)) != STAB_OK) return rc;

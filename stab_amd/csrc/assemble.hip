@@ -128,3 +128,61 @@ __global__ __launch_bounds__(256) void k_assemble(const PointDev *__restrict__ p
 
 template __global__ void k_assemble<double, 8>(const PointDev *, const zd *, const double *, const double *, zd *, int, int);
 template __global__ void k_assemble<float, 8>(const PointDev *, const zd *, const double *, const double *, zf *, int, int);
+
+// The shifted matrix (scaled by dr, dc when `scaled`) for the solve paths,
+// one thread per row: the row's five (BH, VYY, DH) column coefficients are
+// formed once, then the columns are written node by node (consecutive lanes
+// hold consecutive rows, so every store and every D1/D2 load coalesces).
+// Same entries, in the same operation order, as pencil_entry(FORM_SHIFTED).
+// grid: x = chunks of 256 rows, y = point.
+template <typename T>
+__global__ __launch_bounds__(256) void k_assemble_rows(const PointDev *__restrict__ pts, const zd *__restrict__ coef,
+                                                       const double *__restrict__ ops, const double *__restrict__ scales,
+                                                       cx<T> *__restrict__ mats, int scaled) {
+  const PointDev &P = pts[blockIdx.y];
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= P.npad) return;
+  const int ld = P.npad, ny = P.ny, n = P.n;
+  cx<T> *M = mats + P.mat_off;
+  const double *dr = scales + P.scale_off, *dc = dr + P.npad;
+  const double rs = scaled ? dr[r] : 1.0;
+  const int kind = row_kind(P, r);
+  if (kind == ROW_PAD || kind == ROW_DIRICHLET) {
+    const zd d = (kind == ROW_PAD) ? mk(1.0, 0.0) : dirichlet_diag(P, FORM_SHIFTED);
+    for (int c = 0; c < P.npad; ++c) {
+      zd v = (c == r) ? d : czero<double>();
+      if (scaled) v = v * (rs * dc[c]);
+      M[(size_t)c * ld + r] = cvt<T>(v);
+    }
+    return;
+  }
+  const int i = r / 5, p = r - 5 * i;
+  const zd *node = coef + P.coef_off + (size_t)i * NODE_STRIDE;
+  zd bh[5], dh[5];
+  double vyy[5];
+#pragma unroll
+  for (int q = 0; q < 5; ++q) form_coef(P, node, FORM_SHIFTED, p, q, P.shift, bh[q], vyy[q], dh[q]);
+  const double *D1 = ops + P.op_off, *D2 = D1 + (size_t)ny * ny, *dt2w = D2 + (size_t)ny * ny;
+  for (int j = 0; j < ny; ++j) {
+    const double d1 = D1[(size_t)j * ny + i], d2 = D2[(size_t)j * ny + i];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      zd v;
+      if (kind == ROW_GENERIC) {
+        v = bh[q] * d1 - mk(vyy[q] * d2, 0.0);
+      } else if (q < 4) {   // wall energy row (wallt = 2)
+        v = (P.itype == 1) ? bh[q] * d1 - mk(vyy[q] * d1, 0.0) : bh[q] * d1 - mk(vyy[q] * d2, 0.0);
+      } else {
+        v = bh[q] * 0.0 - mk(vyy[q] * dt2w[j], 0.0);
+      }
+      if (i == j) v = v + dh[q];
+      const int c = 5 * j + q;
+      if (scaled) v = v * (rs * dc[c]);
+      M[(size_t)c * ld + r] = cvt<T>(v);
+    }
+  }
+  for (int c = n; c < P.npad; ++c) M[(size_t)c * ld + r] = czero<T>();
+}
+
+template __global__ void k_assemble_rows<double>(const PointDev *, const zd *, const double *, const double *, zd *, int);
+template __global__ void k_assemble_rows<float>(const PointDev *, const zd *, const double *, const double *, zf *, int);

@@ -63,6 +63,7 @@ __device__ void lu_solve(const PointDev &P, const cx<T> *__restrict__ A, const i
     if (ti < bs) {
       zd s = czero<double>();
       const cx<T> *row = Li + (size_t)(k0 / BS) * BS * BS + ti * BS;
+#pragma unroll 4
       for (int k = tq; k <= ti; k += 4) s = cfma(s, cvt<double>(row[k]), y[k0 + k]);
       s.x += __shfl_xor(s.x, 1); s.y += __shfl_xor(s.y, 1);
       s.x += __shfl_xor(s.x, 2); s.y += __shfl_xor(s.y, 2);
@@ -112,6 +113,7 @@ __device__ void lu_solve(const PointDev &P, const cx<T> *__restrict__ A, const i
     if (ti < bs) {
       zd s = czero<double>();
       const cx<T> *row = Ui + (size_t)b * BS * BS + ti * BS;
+#pragma unroll 4
       for (int k = ti + ((tq - ti) & 3); k < bs; k += 4) s = cfma(s, cvt<double>(row[k]), y[k0 + k]);
       s.x += __shfl_xor(s.x, 1); s.y += __shfl_xor(s.y, 1);
       s.x += __shfl_xor(s.x, 2); s.y += __shfl_xor(s.y, 2);

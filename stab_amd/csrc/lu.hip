@@ -485,14 +485,20 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
 }
 
 // Inverses of the diagonal blocks of L (unit lower) and U (upper) of order
-// BS (the last block may be shorter and is stored padded to BS x BS), in
-// FP64, stored as T, row-major: with nblk = ceil(npad / BS), block b of
-// inv(L) at blk_off + b BS^2 and of inv(U) at blk_off + (nblk + b) BS^2.  The
-// per-point allocation is 2 nblk BS^2 elements (blk_elems below).
-// grid: x = block index, y = point.  One column per thread.
+// BS = 64 (the last block may be shorter; entries outside it are stored as
+// zeros), in FP64 arithmetic, stored as T, row-major: with nblk =
+// ceil(npad / BS), block b of inv(L) at blk_off + b BS^2 and of inv(U) at
+// blk_off + (nblk + b) BS^2.  The per-point allocation is 2 nblk BS^2
+// elements (blk_elems above).
+// grid: x = block index, y = point; 256 threads, four lanes per column: lane
+// q of column c sums the terms k = q (mod 4) of each substitution step and
+// keeps the entries x[k][c], k = q (mod 4), in registers; the step's sum is
+// reduced over the four lanes by shuffles, so the 64-step recurrence needs
+// no barrier.  Only the block itself lives in LDS.
 template <typename T, int BS>
-__global__ __launch_bounds__(BS) void k_blkinv(const PointDev *__restrict__ pts, const cx<T> *__restrict__ mats,
-                                               cx<T> *__restrict__ blks) {
+__global__ __launch_bounds__(4 * BS) void k_blkinv(const PointDev *__restrict__ pts, const cx<T> *__restrict__ mats,
+                                                   cx<T> *__restrict__ blks) {
+  static_assert(BS == 64, "four lanes per column of a 64-wide block");
   const PointDev &P = pts[blockIdx.y];
   const int n = P.npad, k0 = blockIdx.x * BS;
   if (k0 >= n) return;
@@ -502,28 +508,56 @@ __global__ __launch_bounds__(BS) void k_blkinv(const PointDev *__restrict__ pts,
   cx<T> *Li = blks + P.blk_off + (size_t)blockIdx.x * BS * BS;
   cx<T> *Ui = blks + P.blk_off + (size_t)(nblk + blockIdx.x) * BS * BS;
   __shared__ zd s_a[BS * BS];     // row-major copy of the block
-  __shared__ zd s_x[BS * BS];     // column t of the inverse being built
-  for (int e = t; e < BS * BS; e += BS) {
+  for (int e = t; e < BS * BS; e += 4 * BS) {
     const int i = e / BS, j = e % BS;
     s_a[e] = (i < bs && j < bs) ? cvt<double>(A[(size_t)(k0 + j) * n + k0 + i]) : czero<double>();
   }
   __syncthreads();
-  const int c = t;
-  // L^-1 column c: forward substitution, unit diagonal
-  for (int i = 0; i < BS; ++i) {
-    zd s = (i == c) ? mk(1.0, 0.0) : czero<double>();
-    if (i > c && i < bs)
-      for (int k = c; k < i; ++k) s = cfms(s, s_a[i * BS + k], s_x[k * BS + c]);
-    s_x[i * BS + c] = (i < c || i >= bs) ? czero<double>() : s;
+  const int c = t >> 2, q = t & 3;
+  zd x[BS / 4];                   // x[m] = entry (4 m + q, c) of the inverse being built
+  // inv(L) column c: x_i = delta_ic - sum_{c <= k < i} L_ik x_k, i > c
+#pragma unroll
+  for (int m = 0; m < BS / 4; ++m) x[m] = czero<double>();
+#pragma unroll
+  for (int mi = 0; mi < BS / 4; ++mi) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = 4 * mi + r;
+      zd s = czero<double>();
+#pragma unroll
+      for (int m = 0; m <= mi; ++m) {
+        const int k = 4 * m + q;
+        if (k < i && k >= c) s = cfma(s, s_a[i * BS + k], x[m]);
+      }
+      s.x += __shfl_xor(s.x, 1); s.y += __shfl_xor(s.y, 1);
+      s.x += __shfl_xor(s.x, 2); s.y += __shfl_xor(s.y, 2);
+      const zd v = (i < c || i >= bs || c >= bs) ? czero<double>() : ((i == c) ? mk(1.0, 0.0) : czero<double>() - s);
+      if (r == q) x[mi] = v;
+    }
   }
-  for (int i = 0; i < BS; ++i) Li[i * BS + c] = cvt<T>(s_x[i * BS + c]);
-  __syncthreads();
-  // U^-1 column c: back substitution
-  for (int i = BS - 1; i >= 0; --i) {
-    zd s = (i == c) ? mk(1.0, 0.0) : czero<double>();
-    if (i < c && c < bs)
-      for (int k = i + 1; k <= c; ++k) s = cfms(s, s_a[i * BS + k], s_x[k * BS + c]);
-    s_x[i * BS + c] = (i > c || c >= bs) ? czero<double>() : cdiv(s, s_a[i * BS + i]);
+#pragma unroll
+  for (int m = 0; m < BS / 4; ++m) Li[(4 * m + q) * BS + c] = cvt<T>(x[m]);
+  // inv(U) column c: x_i = (delta_ic - sum_{i < k <= c} U_ik x_k) / U_ii, i = c .. 0
+#pragma unroll
+  for (int m = 0; m < BS / 4; ++m) x[m] = czero<double>();
+#pragma unroll
+  for (int mi = BS / 4 - 1; mi >= 0; --mi) {
+#pragma unroll
+    for (int r = 3; r >= 0; --r) {
+      const int i = 4 * mi + r;
+      zd s = czero<double>();
+#pragma unroll
+      for (int m = mi; m < BS / 4; ++m) {
+        const int k = 4 * m + q;
+        if (k > i && k <= c) s = cfma(s, s_a[i * BS + k], x[m]);
+      }
+      s.x += __shfl_xor(s.x, 1); s.y += __shfl_xor(s.y, 1);
+      s.x += __shfl_xor(s.x, 2); s.y += __shfl_xor(s.y, 2);
+      const zd rhs = ((i == c) ? mk(1.0, 0.0) : czero<double>()) - s;
+      const zd v = (i > c || c >= bs) ? czero<double>() : cdiv(rhs, s_a[i * BS + i]);
+      if (r == q) x[mi] = v;
+    }
   }
-  for (int i = 0; i < BS; ++i) Ui[i * BS + c] = cvt<T>(s_x[i * BS + c]);
+#pragma unroll
+  for (int m = 0; m < BS / 4; ++m) Ui[(4 * m + q) * BS + c] = cvt<T>(x[m]);
 }

@@ -122,6 +122,7 @@ __device__ void pencil_apply(const PointDev &P, const zd *__restrict__ coef, con
     for (int i = t; i < ny; i += NT) {
       zd a[5], b[5];
       for (int k = 0; k < 5; ++k) { a[k] = czero<double>(); b[k] = czero<double>(); }
+#pragma unroll 4
       for (int j = 0; j < ny; ++j) {
         const double d1 = D1[(size_t)j * ny + i];
         const double d2 = need_d2 ? D2[(size_t)j * ny + i] : 0.0;
