@@ -17,8 +17,9 @@ discrete mode (decaying in the freestream, off the c = 1 continuum).
   chosen  The shift MLP (trained on plain-path solves by the library's own
           GPU kernels, stab_amd/data/shift_mlp.npz) predicts the leading
           eigenvalue; FP32 LU on MFMA at that shift; FP64 residual inverse
-          iteration; FP64 Krylov-Schur from sigma for points whose residual
-          stays above 1e-13 or that land on a non-discrete mode.
+          iteration; FP64 Krylov-Schur for the 4 pairs nearest the predicted
+          shift for points whose residual stays above 1e-13 or that land on
+          a non-discrete mode.
 
 value = chosen-path leading eigenmodes per second, counting a point when its
 eigenvalue equals the plain path's leading eigenvalue to 1e-8 (relative).
@@ -177,8 +178,9 @@ def main():
         t0 = time.perf_counter()
         om = learned.shifts(mlp, feats)
         tp = time.perf_counter() - t0
-        r = S.solve_leading(pts, _lib.MODE_MIXED, _lib.LEAD_INVIT, maxit=args.maxit, fallback_shifts=sigma,
-                            shifts=om)
+        # fallback: FP64 Krylov-Schur for the 4 pairs nearest the predicted shift
+        r = S.solve_leading(pts, _lib.MODE_MIXED, _lib.LEAD_INVIT, maxit=args.maxit, fallback_shifts=om,
+                            shifts=om, nev=4)
         r["stats"]["ms_predict"] = tp * 1e3
         return r
 

@@ -714,14 +714,15 @@ static int solve_run(stab_ctx *c, int npts, const stab_point *pts, int mode, Bat
     if (rc) return rc;
   }
   (void)hipSetDevice(c->device);
+  int nevmax = 0;
+  for (int i = 0; i < npts; ++i) nevmax = std::max(nevmax, pts[i].nev);
+  // basis size scaled to the pairs wanted: 24 / keep 12 up to 4 pairs, 40 / 20 above
   KSParams kp;
-  kp.m = env_int("STAB_KS_M", 40);
-  kp.keep = env_int("STAB_KS_KEEP", 20);
+  kp.m = env_int("STAB_KS_M", nevmax <= 4 ? 24 : 40);
+  kp.keep = env_int("STAB_KS_KEEP", nevmax <= 4 ? 12 : 20);
   kp.maxcycles = env_int("STAB_KS_CYCLES", 10);
   kp.ir = (mode == STAB_MODE_MIXED) ? -env_int("STAB_IR_MAX", 12) : 0;   // adaptive refinement
   kp.tol = 1e-12;
-  int nevmax = 0;
-  for (int i = 0; i < npts; ++i) nevmax = std::max(nevmax, pts[i].nev);
   if (kp.m > KS_MMAX || kp.keep > 32 || kp.keep < nevmax || kp.keep >= kp.m)
     return fail(c, STAB_E_ARG, "Krylov-Schur sizes m=%d keep=%d nev=%d", kp.m, kp.keep, nevmax);
   const bool f32 = (mode == STAB_MODE_MIXED);

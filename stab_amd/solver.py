@@ -231,11 +231,14 @@ class Solver:
 
     def solve_leading(self, points: Sequence[Point], mode: int = _lib.MODE_PLAIN_F64,
                       method: int = _lib.LEAD_KRYLOV, maxit: int = 30, fallback_shifts=None,
-                      want_vectors: bool = False, shifts=None):
+                      want_vectors: bool = False, shifts=None, nev=None):
         """Leading (least stable discrete) eigenmode per point (stab_solve_leading).
-        `shifts` (complex, one per point) replaces the points' shifts."""
+        `shifts` (complex, one per point) replaces the points' shifts, `nev` their
+        pair counts."""
         npts = len(points)
         arr = (Point * npts)(*points)
+        if nev is not None:
+            np.ctypeslib.as_array(arr)["nev"] = int(nev)
         if shifts is not None:
             sh = np.asarray(shifts, dtype=np.complex128)
             assert sh.shape == (npts,)
