@@ -47,6 +47,9 @@ typedef struct { double re, im; } stab_z;
 /* Leading-mode methods (stab_solve_leading) */
 #define STAB_LEAD_KRYLOV 0   /* Krylov-Schur for pt.nev pairs nearest pt.shift, then the leading one */
 #define STAB_LEAD_INVIT  1   /* residual inverse iteration from pt.shift, FP64 Krylov-Schur fallback */
+#define STAB_TRACK       2   /* as STAB_LEAD_INVIT, but the converged eigenvalue nearest pt.shift is
+                                accepted whether discrete or not (mode tracking; the fallback returns
+                                the converged pair nearest the fallback shift) */
 
 /* A mode is discrete when its eigenvector, normalised to a largest entry of
  * modulus 1, stays below STAB_FARFIELD_MAX on the nodes of the outer eighth of
@@ -203,6 +206,10 @@ size_t stab_evec_bytes(int npts, const stab_point *pts);
  *     lands on a mode that is not discrete, is solved again by
  *     STAB_LEAD_KRYLOV in FP64 from fallback_shift[p] (pt.shift if NULL) and
  *     pt.nev, and reports STAB_INFO_FALLBACK.
+ *   method STAB_TRACK: as STAB_LEAD_INVIT with "nearest the shift" in place of
+ *     "leading discrete": the eigenvalue selection of getab.f90:160-198 /
+ *     getalpha.f90:139-145 (nearest to a guess or an extrapolation) without
+ *     the full spectrum.
  *   eval/resid/farfield/iters/info: npts entries (iters = operator solves);
  *   evec: NULL or sum over points of nvec (5 ny temporal, 10 ny spatial with
  *     z = [phi; alpha phi]), normalised as getevec.f90:179-189.

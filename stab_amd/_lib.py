@@ -17,7 +17,7 @@ MODE_PLAIN_F64 = 0
 MODE_MIXED = 1
 MAT_A0, MAT_B0, MAT_C0, MAT_C1, MAT_C2, MAT_SHIFTED = range(6)
 INFO_OK, INFO_SINGULAR, INFO_NOCONV, INFO_FALLBACK = 0, 1, 2, 3
-LEAD_KRYLOV, LEAD_INVIT = 0, 1
+LEAD_KRYLOV, LEAD_INVIT, TRACK = 0, 1, 2
 FARFIELD_MAX, CONTINUUM_BAND = 0.1, 0.01
 
 EXPORTS = ("stab_ctx_create", "stab_ctx_destroy", "stab_last_error", "stab_profile_set", "stab_grid",

@@ -920,7 +920,8 @@ static int gather_lead_vecs(stab_ctx *c, int npts, Batch &B, const std::vector<i
 
 // Krylov-Schur for the pt.nev pairs nearest pt.shift, then the leading
 // discrete pair among those with residual below 1e-10.
-static int lead_krylov(stab_ctx *c, int npts, const stab_point *pts, int mode, const LeadOut &o, stab_stats *stats) {
+static int lead_krylov(stab_ctx *c, int npts, const stab_point *pts, int mode, const LeadOut &o, stab_stats *stats,
+                       bool nearest = false) {
   Batch B;
   RunInfo ri;
   int rc;
@@ -943,7 +944,12 @@ static int lead_krylov(stab_ctx *c, int npts, const stab_point *pts, int mode, c
     int best = -1;
     for (int k = 0; k < pts[i].nev && !inf[i]; ++k) {
       const size_t e = (size_t)i * KS_NEVMAX + k;
-      if (!(rs[e] < 1e-10) || !lead_discrete(pts[i], ev[e], ff[e])) continue;
+      if (!(rs[e] < 1e-10)) continue;
+      if (nearest) {             // pairs come sorted by distance to the shift
+        best = k;
+        break;
+      }
+      if (!lead_discrete(pts[i], ev[e], ff[e])) continue;
       if (best < 0 || lead_key(pts[i], ev[e]) > lead_key(pts[i], ev[(size_t)i * KS_NEVMAX + best])) best = k;
     }
     sel[i] = best;
@@ -966,7 +972,7 @@ static int lead_krylov(stab_ctx *c, int npts, const stab_point *pts, int mode, c
 
 // Factor at pt.shift and run residual inverse iteration (k_invit).
 static int lead_invit(stab_ctx *c, int npts, const stab_point *pts, int mode, int maxit, const LeadOut &o,
-                      std::vector<int> &ok, stab_stats *stats) {
+                      std::vector<int> &ok, stab_stats *stats, bool nearest = false) {
   if (mode != STAB_MODE_PLAIN_F64 && mode != STAB_MODE_MIXED) return fail(c, STAB_E_ARG, "unknown mode %d", mode);
   if (maxit < 1 || maxit > 1000) return fail(c, STAB_E_ARG, "maxit %d outside 1..1000", maxit);
   for (int i = 0; i < npts; ++i) {
@@ -1025,7 +1031,7 @@ static int lead_invit(stab_ctx *c, int npts, const stab_point *pts, int mode, in
     if (o.resid) o.resid[i] = rs[i];
     if (o.ff) o.ff[i] = ff[i];
     if (o.iters) o.iters[i] = it[i];
-    ok[i] = !inf[i] && fl[i] == 0 && lead_discrete(pts[i], ev[i], ff[i]);
+    ok[i] = !inf[i] && fl[i] == 0 && (nearest || lead_discrete(pts[i], ev[i], ff[i]));
     o.info[i] = inf[i] ? STAB_INFO_SINGULAR : (ok[i] ? STAB_INFO_OK : STAB_INFO_NOCONV);
     found += ok[i];
   }
@@ -1056,9 +1062,10 @@ int stab_solve_leading(stab_ctx *c, int npts, const stab_point *pts, int mode, i
   if (npts <= 0 || !pts || !eval || !info) return fail(c, STAB_E_ARG, "npts/pts/eval/info");
   const LeadOut o{eval, evec, resid, farfield, iters, info};
   if (method == STAB_LEAD_KRYLOV) return lead_krylov(c, npts, pts, mode, o, stats);
-  if (method != STAB_LEAD_INVIT) return fail(c, STAB_E_ARG, "unknown method %d", method);
+  if (method != STAB_LEAD_INVIT && method != STAB_TRACK) return fail(c, STAB_E_ARG, "unknown method %d", method);
+  const bool nearest = (method == STAB_TRACK);
   std::vector<int> ok;
-  int rc = lead_invit(c, npts, pts, mode, maxit, o, ok, stats);
+  int rc = lead_invit(c, npts, pts, mode, maxit, o, ok, stats, nearest);
   if (rc != STAB_OK) return rc;
   // Fallback: points whose iteration missed the residual bound or landed on a
   // mode that is not discrete are solved by the plain FP64 Krylov-Schur path
@@ -1083,7 +1090,7 @@ int stab_solve_leading(stab_ctx *c, int npts, const stab_point *pts, int mode, i
   std::vector<stab_z> vec(evec ? soff[nf] : 0);
   const LeadOut fo{ev.data(), evec ? vec.data() : nullptr, rs.data(), ff.data(), it.data(), inf.data()};
   stab_stats fst;
-  if ((rc = lead_krylov(c, nf, sub.data(), STAB_MODE_PLAIN_F64, fo, &fst)) != STAB_OK) return rc;
+  if ((rc = lead_krylov(c, nf, sub.data(), STAB_MODE_PLAIN_F64, fo, &fst, nearest)) != STAB_OK) return rc;
   for (int k = 0; k < nf; ++k) {
     const int i = idx[k];
     eval[i] = ev[k];
