@@ -13,6 +13,13 @@
 ! gives that shift (the value the reference's getevec -v reads from its
 ! heredoc, thesis/TStest/run.sh:13-17) and the number of pairs (default 8).
 ! Without it the shift is 0.35 alpha (temporal) or omega / 0.35 (spatial).
+! itype 7 is the multiple-temporal sweep of stab.f90:84-86 / mtemporal.f90:
+! no alpha/beta lines and no index (profile.0), then the lines
+! "alpha_min alpha_max alpha_inc" and "beta_min beta_max beta_inc"; the
+! points (alpha outer, beta inner, mtemporal.f90:25-38) are solved in one
+! batched call and each writes eig.<iver> with the evec.dat records.  For it
+! the optional last line is a phase-speed guess "c_r c_i [nev]" (shift =
+! alpha c; default c = 0.35).
 ! Record 4 holds the computed eigenvalues followed by huge placeholders, and
 ! record 5 their eigenvectors followed by zero columns, so both records keep
 ! the reference's length and getevec reads the file unchanged.
@@ -30,7 +37,13 @@ program stab_gpu
   type(stab_point) :: pt(1)
   type(c_ptr) :: ctx
   integer(c_int) :: rc, mode
-  integer :: nym, n, nvec, nmax, i, k
+  integer :: nym, n, nvec, nmax, i, k, na, nb, ia, ib, npts, iver
+  real(8) :: amin, amax, ainc, bmin, bmax, binc
+  complex(8) :: cguess
+  type(stab_point), allocatable :: pts(:)
+  complex(8), allocatable :: evs(:), vecs(:)
+  real(8), allocatable :: ress(:)
+  integer(c_int32_t), allocatable :: infs(:)
   real(8), allocatable, target :: ym(:), vt(:,:), y(:), eta(:), deta(:), d2eta(:)
   complex(8), allocatable :: eval(:), evec(:), evout(:), vecout(:,:)
   real(8), allocatable :: resid(:)
@@ -63,18 +76,41 @@ program stab_gpu
     read (*, *) orr, oi
     read (*, *) br, bi
     omega = cmplx(orr, oi, kind=8); beta = cmplx(br, bi, kind=8)
+  else if (itype == 7) then
+    continue                                     ! input.f90:103-107: nothing more
   else
     write (*, *) 'itype ', itype, ' is outside the Chebyshev path of stab_gpu'
     call exit(2)
   end if
-  read (*, *) ind
+  ind = 0
+  if (itype /= 7) read (*, *) ind
   if (itype == 2) read (*, *) x
+  if (itype == 7) then
+    read (*, *) amin, amax, ainc                 ! mtemporal.f90:20-23
+    read (*, *) bmin, bmax, binc
+  end if
   if (curve /= 0) then
     write (*, *) 'curve /= 0 (wall curvature) is outside this path'
     call exit(2)
   end if
   ! optional shift line
   nev = 8
+  cguess = cmplx(0.35d0, 0.0d0, kind=8)
+  if (itype == 7) then
+    read (*, *, iostat=ios) sr, si, rnev
+    if (ios == 0) then
+      cguess = cmplx(sr, si, kind=8)
+      nev = int(rnev)
+    else
+      read (*, *, iostat=ios) sr, si
+      if (ios == 0) cguess = cmplx(sr, si, kind=8)
+    end if
+    nev = max(1, min(nev, STAB_MAX_NEV))
+    call makename('profile', ind, fname)
+    call read_profile(trim(fname), nym, ym, vt)
+    call sweep_temporal()
+    stop
+  end if
   if (itype == 1) then
     shift = cmplx(0.35d0 * real(alpha), 0.0d0, kind=8)
   else
@@ -141,6 +177,65 @@ program stab_gpu
   close (77)
 
 contains
+
+  ! mtemporal.f90:25-38 batched: all (alpha, beta) points in one stab_solve,
+  ! then eig.<iver> per point with the records of temporal.f90:883-890.
+  subroutine sweep_temporal()
+    na = max(nint((amax - amin) / ainc), 1)
+    nb = max(nint((bmax - bmin) / binc), 1)
+    npts = na * nb
+    allocate (pts(npts))
+    n = 5 * ny
+    iver = 0
+    do ia = 1, na
+      do ib = 1, nb
+        iver = iver + 1
+        pts(iver)%itype = 1; pts(iver)%ny = ny; pts(iver)%mattyp = mattyp; pts(iver)%wallt = wallt
+        pts(iver)%top = top; pts(iver)%profile = 0; pts(iver)%nev = nev; pts(iver)%ider = merge(1, 0, ider)
+        pts(iver)%Ma = Ma; pts(iver)%Re = Re; pts(iver)%Pr = Pr; pts(iver)%Te = Te
+        pts(iver)%yi = yi; pts(iver)%ymax = ymax
+        pts(iver)%alpha = cmplx(amin + dble(ia - 1) * ainc, 0.0d0, kind=8)
+        pts(iver)%beta = cmplx(bmin + dble(ib - 1) * binc, 0.0d0, kind=8)
+        pts(iver)%omega = (0.0d0, 0.0d0)
+        pts(iver)%shift = pts(iver)%alpha * cguess
+      end do
+    end do
+    rc = stab_ctx_create(ctx, 0)
+    if (rc /= STAB_OK) then
+      write (*, *) 'stab_gpu: no usable HIP device (status ', rc, ')'
+      call exit(3)
+    end if
+    rc = stab_profile_set(ctx, 0, nym, ym, vt)
+    call check(rc)
+    allocate (evs(STAB_MAX_NEV * npts), vecs(n * STAB_MAX_NEV * npts), ress(STAB_MAX_NEV * npts), infs(npts))
+    call get_mode(mode)
+    rc = stab_solve(ctx, npts, pts, mode, evs, vecs, ress, infs, c_null_ptr)
+    call check(rc)
+    allocate (y(ny), eta(ny), deta(ny), d2eta(ny))
+    rc = stab_grid(ctx, pts(1), c_loc(y), c_loc(eta), c_loc(deta), c_loc(d2eta), c_null_ptr, c_null_ptr, c_null_ptr)
+    call check(rc)
+    rc = stab_ctx_destroy(ctx)
+    allocate (evout(n), vecout(n, n))
+    do iver = 1, npts
+      write (*, '(2(1x,i3,1x),4(1pe13.6,1x))') (iver - 1) / nb + 1, mod(iver - 1, nb) + 1, &
+        real(pts(iver)%alpha), aimag(pts(iver)%alpha), real(pts(iver)%beta), aimag(pts(iver)%beta)
+      evout = cmplx(huge_ev, 0.0d0, kind=8)
+      vecout = (0.0d0, 0.0d0)
+      do k = 1, nev
+        evout(k) = evs((iver - 1) * STAB_MAX_NEV + k)
+        vecout(:, k) = vecs((iver - 1) * n * STAB_MAX_NEV + (k - 1) * n + 1:(iver - 1) * n * STAB_MAX_NEV + k * n)
+      end do
+      if (infs(iver) /= STAB_INFO_OK) write (*, *) 'stab_gpu: point ', iver, ' info = ', infs(iver)
+      call makename('eig', iver, fname)
+      open (unit=77, file=trim(fname), form='unformatted', status='unknown')
+      write (77) ind, ny, 5, 1, ievec, curve, top, wall, wallt, ider
+      write (77) (0.0d0, 0.0d0), pts(iver)%alpha, pts(iver)%beta, Re, Ma, Pr
+      write (77) x, y, eta, deta, d2eta, yi, ymax
+      write (77) evout
+      write (77) vecout
+      close (77)
+    end do
+  end subroutine sweep_temporal
 
   subroutine check(code)
     integer(c_int), intent(in) :: code

@@ -339,3 +339,24 @@ def test_discrete_rule_rejects_continuum_and_growing_farfield():
     assert not so.is_discrete(1, 0.1 - 0.001j, so.farfield_ratio(decay, ny), alpha_r=0.1)   # c = 1
     assert not so.is_discrete(2, 0.1 - 0.001j, so.farfield_ratio(decay, ny), omega_r=0.1)   # c = 1
     assert not so.is_discrete(2, -0.01 - 0.9j, so.farfield_ratio(decay, ny), omega_r=0.1)   # upstream
+
+
+def test_ndiff_equivalent_on_goldens(ts_profile, tmp_path):
+    """stab_amd.ndiff: the oracle's thesis eigenfunction text agrees with
+    time.ref under -abserr 1e-8 and a perturbation of 2e-8 in one field is
+    caught at the right place."""
+    from stab_amd import ndiff
+    ref = open(golden("thesis_TStest", "time.ref")).read()
+    ok, worst, bad = ndiff.compare(ref, ref, abserr=1e-8)
+    assert ok and worst == 0.0
+    rows = ref.splitlines()
+    k = next(i for i, l in enumerate(rows) if not l.startswith("#"))
+    f = rows[k + 3].split()
+    f[4] = "%.13E" % (float(f[4]) + 2e-8)
+    rows[k + 3] = " ".join(f)
+    ok, worst, bad = ndiff.compare(ref, "\n".join(rows), abserr=1e-8)
+    assert not ok and bad == (3, 4) and abs(worst - 2e-8) < 1e-12
+    p1, p2 = tmp_path / "a", tmp_path / "b"
+    p1.write_text(ref)
+    p2.write_text(ref)
+    assert ndiff.main(["-abserr", "1e-8", str(p1), str(p2)]) == 0
