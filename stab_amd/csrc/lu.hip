@@ -85,9 +85,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
   constexpr int KB = 64, GW = lu_gw<T>(), UP = GW + 1, NPB = KB / NB;
   static_assert(KB % NB == 0, "panel width must divide the block width");
 
-  __shared__ C s_prow[NB], s_crow[NB];
-  __shared__ T s_rv[NW];
-  __shared__ int s_ri[NW];
+  // double-buffered by column parity: a column's slots are rewritten two
+  // columns later, after a barrier every wave passes only once it has read them
+  __shared__ C s_prow2[2][NB], s_crow2[2][NB];
+  __shared__ T s_rv2[2][NW];
+  __shared__ int s_ri2[2][NW];
   __shared__ C s_blk[NB * NB];
   __shared__ C s_linv[NPB][NB * NB];      // inv(L11) of each panel of the block
   __shared__ int s_piv[KB];
@@ -120,6 +122,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
 #pragma unroll 1
       for (int c = 0; c < NB; ++c) {
         const int col = k0 + c;
+        C *s_prow = s_prow2[c & 1], *s_crow = s_crow2[c & 1];
+        T *s_rv = s_rv2[c & 1];
+        int *s_ri = s_ri2[c & 1];
         T best = (T)-1;
         int bi = 0x7fffffff;
 #pragma unroll
@@ -196,7 +201,6 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
             }
           }
         }
-        __syncthreads();
       }
       // ---- write the factored panel back; stage the diagonal block
 #pragma unroll
