@@ -240,7 +240,7 @@ def main(argv=None):
     ap.add_argument("--eval-points", type=int, default=8192)
     ap.add_argument("--ny-train", type=int, default=96)
     ap.add_argument("--ny-eval", type=int, default=128)
-    ap.add_argument("--steps", type=int, default=12000)
+    ap.add_argument("--steps", type=int, default=60000)
     ap.add_argument("--seed", type=int, default=11)
     ap.add_argument("--workers", type=int, default=8)
     ap.add_argument("--out", default=WEIGHTS)
