@@ -38,16 +38,105 @@ struct KSState {
 
 // ---------------------------------------------------------------------------
 // y <- M^-1 y with the factors of k_lu and the 64 x 64 diagonal-block
-// inverses of k_blkinv (scales applied by the caller).  Per block of 64 rows:
-// y_top <- inv(D) y_top as a GEMV (4 threads per row), then the rows below
-// (forward) or above (backward) are updated with independent accumulators
-// for all rows a thread owns.  y, work in LDS (npad), tmp in LDS (>= 64).
+// inverses of k_blkinv (scales applied by the caller).  The solve reads the
+// whole factor once and is bound by how many bytes one work-group keeps in
+// flight, so every load is 16 bytes wide (two FP32 or one FP64 complex matrix
+// entries of consecutive rows) and each lane issues 16 of them before using
+// any (64 KB in flight per work-group).  Per block of 64 rows:
+//   1. y_top <- inv(D) y_top: lane (i, q) sums columns 16q..16q+15 of row i
+//      of the column-major inverse (a wave reads 64 consecutive entries per
+//      load); the four partial sums meet in tmp (NT = 256 entries);
+//   2. the rows below (forward) or above (backward): lane (g, q) of a row
+//      group g (E rows, E = 16 bytes / entry size) loads 16 columns of
+//      quarter q and the quarters are summed by shuffles.
+// Columns of a partial last block (bs < 64) beyond bs are never loaded.
+// y, work in LDS (npad); tmp in LDS (NT entries).
 // ---------------------------------------------------------------------------
 #define SOLVE_BS 64
+template <typename T> struct Row16;
+template <> struct Row16<float> {
+  typedef float4 V;
+  static constexpr int E = 2;
+  __device__ static zd get(const V &v, int e) { return e == 0 ? mk((double)v.x, (double)v.y) : mk((double)v.z, (double)v.w); }
+};
+template <> struct Row16<double> {
+  typedef double2 V;
+  static constexpr int E = 1;
+  __device__ static zd get(const V &v, int) { return mk(v.x, v.y); }
+};
+
+// tmp[0..bs) <- inv(D) y[k0..k0+bs) with D the diagonal block at inv (column
+// major, entries beyond bs zero), tmp[bs..64) <- 0; y[k0..k0+bs) <- the same.
+// Branch-free: indexes past bs are clamped to bs - 1 and meet zero entries.
+// Two barriers.
 template <typename T, int NT>
-__device__ void lu_solve(const PointDev &P, const cx<T> *__restrict__ A, const int *__restrict__ ipiv,
+__device__ inline void solve_diag(const cx<T> *__restrict__ inv, int k0, int bs, zd *y, zd *tmp) {
+  constexpr int BS = SOLVE_BS;
+  const int t = threadIdx.x, i = t & (BS - 1), q = t >> 6;
+  const cx<T> *col = inv + (size_t)(q * 16) * BS + i;
+  cx<T> v[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) v[j] = col[j * BS];
+  zd s = czero<double>();
+#pragma unroll
+  for (int j = 0; j < 16; ++j) s = cfma(s, cvt<double>(v[j]), y[k0 + min(q * 16 + j, bs - 1)]);
+  tmp[t] = s;
+  __syncthreads();
+  if (t < BS) {
+    const zd r = tmp[t] + tmp[BS + t] + tmp[2 * BS + t] + tmp[3 * BS + t];
+    tmp[t] = (t < bs) ? r : czero<double>();
+    if (t < bs) y[k0 + t] = r;
+  }
+  __syncthreads();
+}
+
+// y[r0..r1) -= A[r0..r1, k0..k0+bs) tmp[0..bs); r0, r1 multiples of E.
+// Branch-free: column indexes past bs are clamped to bs - 1 (tmp is zero
+// there) and rows past r1 to the last row group (not written back).
+template <typename T, int NT>
+__device__ inline void solve_update(const cx<T> *__restrict__ A, int lda, int k0, int bs, int r0, int r1, zd *y,
+                                    const zd *tmp) {
+  typedef Row16<T> R;
+  typedef typename R::V V;
+  constexpr int E = R::E, G = NT / 4;        // G row groups of E rows per pass
+  const int t = threadIdx.x, q = t & 3, g = t >> 2;
+  const int ldv = lda / E;
+  const V *base = (const V *)A;
+  int coff[16];                              // this lane's 16 columns, in V units
+#pragma unroll
+  for (int j = 0; j < 16; ++j) coff[j] = (k0 + min(q * 16 + j, bs - 1)) * ldv;
+#pragma unroll 1
+  for (int rb = r0; rb < r1; rb += G * E) {
+    const int r = rb + g * E, rl = min(r, r1 - E);
+    V v[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) v[j] = base[coff[j] + rl / E];
+    zd a[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) a[e] = czero<double>();
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const zd mj = tmp[q * 16 + j];
+#pragma unroll
+      for (int e = 0; e < E; ++e) a[e] = cfma(a[e], R::get(v[j], e), mj);
+    }
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      a[e].x += __shfl_xor(a[e].x, 1); a[e].y += __shfl_xor(a[e].y, 1);
+      a[e].x += __shfl_xor(a[e].x, 2); a[e].y += __shfl_xor(a[e].y, 2);
+    }
+    if (r < r1 && q == 0) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) y[r + e] = y[r + e] - a[e];
+    }
+  }
+}
+
+template <typename T, int NT>
+__device__ __forceinline__ void lu_solve(const PointDev &P, const cx<T> *__restrict__ A, const int *__restrict__ ipiv,
                          const cx<T> *__restrict__ blk, zd *y, zd *tmp, zd *work) {
-  constexpr int BS = SOLVE_BS, RS = 6;      // npad <= RS * NT
+  static_assert(NT == 4 * SOLVE_BS, "four lanes per row of a 64-row block");
+  constexpr int BS = SOLVE_BS;
   const int n = P.npad, lda = P.npad, t = threadIdx.x;
   const int *perm = ipiv + n;
   const int nblk = (n + BS - 1) / BS;
@@ -57,105 +146,21 @@ __device__ void lu_solve(const PointDev &P, const cx<T> *__restrict__ A, const i
   __syncthreads();
   for (int r = t; r < n; r += NT) y[r] = work[r];
   __syncthreads();
-  const int ti = t >> 2, tq = t & 3;         // row of the block GEMV, quarter of the columns
-  for (int k0 = 0; k0 < n; k0 += BS) {
-    const int bs = min(BS, n - k0);
-    if (ti < bs) {
-      zd s = czero<double>();
-      const cx<T> *row = Li + (size_t)(k0 / BS) * BS * BS + ti * BS;
-#pragma unroll 4
-      for (int k = tq; k <= ti; k += 4) s = cfma(s, cvt<double>(row[k]), y[k0 + k]);
-      s.x += __shfl_xor(s.x, 1); s.y += __shfl_xor(s.y, 1);
-      s.x += __shfl_xor(s.x, 2); s.y += __shfl_xor(s.y, 2);
-      if (tq == 0) tmp[ti] = s;
+  for (int b = 0; b < nblk; ++b) {
+    const int k0 = b * BS, bs = min(BS, n - k0);
+    solve_diag<T, NT>(Li + (size_t)b * BS * BS, k0, bs, y, tmp);
+    if (k0 + bs < n) {
+      solve_update<T, NT>(A, lda, k0, bs, k0 + bs, n, y, tmp);
+      __syncthreads();
     }
-    __syncthreads();
-    if (bs == BS) {
-      // rows below the block, one at a time, eight column loads in flight
-#pragma unroll 1
-      for (int j = 0; j < RS; ++j) {
-        const int r = k0 + BS + t + j * NT;
-        if (r < n) {
-          zd a = y[r];
-          const cx<T> *col = A + (size_t)k0 * lda + r;
-#pragma unroll 4
-          for (int c = 0; c < BS; ++c) a = cfms(a, cvt<double>(col[(size_t)c * lda]), tmp[c]);
-          y[r] = a;
-        }
-      }
-    } else {
-      zd acc[RS];
-#pragma unroll
-      for (int j = 0; j < RS; ++j) {
-        const int r = k0 + bs + t + j * NT;
-        acc[j] = (r < n) ? y[r] : czero<double>();
-      }
-      for (int c = 0; c < bs; ++c) {
-        const zd tc = tmp[c];
-        const cx<T> *col = A + (size_t)(k0 + c) * lda;
-#pragma unroll
-        for (int j = 0; j < RS; ++j) {
-          const int r = k0 + bs + t + j * NT;
-          if (r < n) acc[j] = cfms(acc[j], cvt<double>(col[r]), tc);
-        }
-      }
-#pragma unroll
-      for (int j = 0; j < RS; ++j) {
-        const int r = k0 + bs + t + j * NT;
-        if (r < n) y[r] = acc[j];
-      }
-    }
-    if (t < bs) y[k0 + t] = tmp[t];
-    __syncthreads();
   }
   for (int b = nblk - 1; b >= 0; --b) {
     const int k0 = b * BS, bs = min(BS, n - k0);
-    if (ti < bs) {
-      zd s = czero<double>();
-      const cx<T> *row = Ui + (size_t)b * BS * BS + ti * BS;
-#pragma unroll 4
-      for (int k = ti + ((tq - ti) & 3); k < bs; k += 4) s = cfma(s, cvt<double>(row[k]), y[k0 + k]);
-      s.x += __shfl_xor(s.x, 1); s.y += __shfl_xor(s.y, 1);
-      s.x += __shfl_xor(s.x, 2); s.y += __shfl_xor(s.y, 2);
-      if (tq == 0) tmp[ti] = s;
+    solve_diag<T, NT>(Ui + (size_t)b * BS * BS, k0, bs, y, tmp);
+    if (k0 > 0) {
+      solve_update<T, NT>(A, lda, k0, bs, 0, k0, y, tmp);
+      __syncthreads();
     }
-    __syncthreads();
-    if (bs == BS) {
-#pragma unroll 1
-      for (int j = 0; j < RS; ++j) {
-        const int r = t + j * NT;
-        if (r < k0) {
-          zd a = y[r];
-          const cx<T> *col = A + (size_t)k0 * lda + r;
-#pragma unroll 4
-          for (int c = 0; c < BS; ++c) a = cfms(a, cvt<double>(col[(size_t)c * lda]), tmp[c]);
-          y[r] = a;
-        }
-      }
-    } else {
-      zd acc[RS];
-#pragma unroll
-      for (int j = 0; j < RS; ++j) {
-        const int r = t + j * NT;
-        acc[j] = (r < k0) ? y[r] : czero<double>();
-      }
-      for (int c = 0; c < bs; ++c) {
-        const zd tc = tmp[c];
-        const cx<T> *col = A + (size_t)(k0 + c) * lda;
-#pragma unroll
-        for (int j = 0; j < RS; ++j) {
-          const int r = t + j * NT;
-          if (r < k0) acc[j] = cfms(acc[j], cvt<double>(col[r]), tc);
-        }
-      }
-#pragma unroll
-      for (int j = 0; j < RS; ++j) {
-        const int r = t + j * NT;
-        if (r < k0) y[r] = acc[j];
-      }
-    }
-    if (t < bs) y[k0 + t] = tmp[t];
-    __syncthreads();
   }
 }
 
@@ -203,7 +208,7 @@ struct OpLds {
   __device__ OpLds(zd *base, const PointDev &P, int npadmax, int nymax, int full, int nvecmax) {
     red = base;
     tmp = red + 64;
-    y = tmp + 64;
+    y = tmp + 256;
     z = y + npadmax;
     if (full) {
       rhs = z + nvecmax;
@@ -218,7 +223,7 @@ struct OpLds {
 };
 
 __host__ __device__ inline size_t op_lds_bytes(int npadmax, int nymax, int full, int nvecmax) {
-  return sizeof(zd) * (128 + (size_t)npadmax + (size_t)nvecmax +
+  return sizeof(zd) * (320 + (size_t)npadmax + (size_t)nvecmax +
                        (full ? 2 * (size_t)npadmax + 10 * (size_t)nymax : 0));
 }
 
@@ -323,7 +328,7 @@ __device__ inline double hash01(unsigned a, unsigned b) {
 // Arnoldi expansion from column j0 to m (CGS2).  V column-major, ld = ldv.
 // ---------------------------------------------------------------------------
 template <typename T, int NB, int NT, bool FULL>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FULL ? 2 : 4))) void k_arnoldi(const PointDev *__restrict__ pts, const cx<T> *__restrict__ mats,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FULL ? 2 : 3))) void k_arnoldi(const PointDev *__restrict__ pts, const cx<T> *__restrict__ mats,
                                                 const int *__restrict__ pivs, const cx<T> *__restrict__ blks,
                                                 const double *__restrict__ scales, const zd *__restrict__ coef,
                                                 const double *__restrict__ ops, zd *__restrict__ Vall,

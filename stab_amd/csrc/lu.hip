@@ -10,8 +10,9 @@
 // forming its rows of U as inv(L11) * A12.  When the block column is done:
 //   * its net row permutation is applied to the columns left of it and to the
 //     trailing columns (one pass each),
-//   * U12 = inv(L11) A12 by blocked forward substitution in LDS, with the
-//     16 x 16 inverses kept from the panels, and
+//   * X = inv(L11) is formed once (four lanes per column, shuffles) and
+//     stored as the block's inverse for the triangular solves, and U12 = X A12
+//     runs on the matrix cores, and
 //   * A22 -= L21 U12 with depth KB on the matrix cores, complex arithmetic as
 //     2 x 2 real blocks, [Cr; Ci] -= [[Lr, -Li], [Li, Lr]] [Ur; Ui], i.e.
 //     four real MFMA products per tile, computed transposed (the MFMA row index
@@ -20,8 +21,8 @@
 // The trailing matrix is read and written once per 64 columns.  P A = L U
 // with P given by perm[] as well as by ipiv[] (L is stored in final row
 // order).  Outputs: L and U in place, ipiv (absolute row indices, as ZGETRF),
-// perm (ipiv + n) and info.  k_blkinv then inverts the 64 x 64 diagonal blocks
-// used by the blocked triangular solves.
+// perm (ipiv + n), info and the inverses of the 64 x 64 diagonal blocks of L;
+// k_blkinv then inverts those of U.
 #include "common.h"
 
 // 16 x 16 x 4 MFMA: D[m][n] += sum_k A[m][k] B[k][n]; lane l supplies
@@ -347,8 +348,47 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
         __syncthreads();
       }
     }
-    // ---- trailing columns [kend, n), GW at a time: permute, solve for U12,
-    // then A22 -= L21 U12 on the matrix cores
+    // ---- X = inv(L11) (unit lower, zero outside kb x kb) in T arithmetic,
+    // column-major into this block's slot of the diagonal-block inverses
+    // (read by the triangular solves, and below by U12 = X A12).  Four lanes
+    // per column: lane q of column c keeps x[m] = X(4 m + q, c) in registers
+    // and sums the terms k = q (mod 4) of each substitution step; the sums meet
+    // by shuffles.  The strictly lower triangle of L11 is staged in s_u, packed
+    // by rows (row i at i (i - 1) / 2; 2016 entries, which s_u holds for both
+    // strip widths), with zeros in rows and columns past kb.
+    C *X = blks + P.blk_off + (size_t)(K0 / KB) * KB * KB;
+    static_assert(KB * (KB - 1) / 2 <= KB * UP, "packed L11 fits in s_u");
+    for (int e = t; e < KB * KB; e += NT) {
+      const int i = e % KB, k = e / KB;
+      if (k < i) s_u[i * (i - 1) / 2 + k] = (i < kb) ? A[(size_t)(K0 + k) * lda + K0 + i] : czero<T>();
+    }
+    __syncthreads();
+    if (t < 4 * KB) {
+      const int c = t >> 2, q = t & 3;
+      C x[KB / 4];
+#pragma unroll
+      for (int m = 0; m < KB / 4; ++m) x[m] = czero<T>();
+#pragma unroll
+      for (int mi = 0; mi < KB / 4; ++mi) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = 4 * mi + r;
+          C s = czero<T>();
+#pragma unroll
+          for (int m = 0; m <= mi; ++m)     // entry (i, 4 m + q) of L11 when 4 m + q < i
+            s = cfma(s, (m < mi || q < r) ? s_u[i * (i - 1) / 2 + 4 * m + q] : czero<T>(), x[m]);
+          s.x += __shfl_xor(s.x, 1); s.y += __shfl_xor(s.y, 1);
+          s.x += __shfl_xor(s.x, 2); s.y += __shfl_xor(s.y, 2);
+          const C v = (i < c || i >= kb || c >= kb) ? czero<T>() : ((i == c) ? mk<T>(1, 0) : czero<T>() - s);
+          if (r == q) x[mi] = v;
+        }
+      }
+#pragma unroll
+      for (int m = 0; m < KB / 4; ++m) X[c * KB + 4 * m + q] = x[m];
+    }
+    __syncthreads();
+    // ---- trailing columns [kend, n), GW at a time: permute, U12 = X A12 and
+    // A22 -= L21 U12, both on the matrix cores
     if (kend < n) {
       typedef typename Mfma<T>::v4 v4;
       constexpr int CT = GW / 16, RT = 2;
@@ -367,46 +407,55 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
           const int o = e % bnout, j = e / bnout;
           A[(size_t)(c0 + j) * lda + s_outrow[o]] = A[(size_t)(c0 + j) * lda + s_outsrc[o]];
         }
-        // U12 = inv(L11) A12 by 16-row blocks: apply the panel's inverse,
-        // then eliminate the block from the rows below it
-        for (int b = 0; b < kb / NB; ++b) {
-          const C *Lb = s_linv[b];
-          constexpr int MAXU = (NB * GW + NT - 1) / NT;
-          C u[MAXU];
+        // U12 = X S (S = the staged rows) on the matrix cores, transposed as
+        // the update below: wave w takes column tile w % CTN and the row tiles
+        // it owns (four per column tile, split over NW / CTN waves so that the
+        // triangular work, 4 (rt + 1) depth-4 steps for row tile rt, balances)
+        {
+          constexpr int CTN = GW / 16, NWC = NW / CTN;
+          static_assert(NWC == 1 || NWC == 2 || NWC == 4, "waves per U12 column tile");
+          // X is the same for every strip; an opaque copy of its address keeps
+          // the compiler from hoisting its loads out of the strip loop, where
+          // they would stay live across the update and spill
+          const C *Xs = X;
+          asm volatile("" : "+s"(Xs));
+          const int ct = wv % CTN, h = wv / CTN;
+          const bool colok = ct * 16 < gw;
+          v4 ur[4], ui[4];
 #pragma unroll
-          for (int q = 0; q < MAXU; ++q) {
-            const int e = t + q * NT;
-            if (e < NB * gw) {
-              const int i = e / gw, j = e - (e / gw) * gw;
-              C s = czero<T>();
-              for (int k = 0; k <= i; ++k) s = cfma(s, Lb[i * NB + k], s_u[(b * NB + k) * UP + j]);
-              u[q] = s;
+          for (int rt = 0; rt < 4; ++rt) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) { ur[rt][i] = (T)0; ui[rt][i] = (T)0; }
+            const int own = (NWC == 1) ? 0 : (NWC == 2 ? ((rt == 0 || rt == 3) ? 0 : 1) : rt);
+            if (own != h || !colok) continue;
+            // X(i, k) = 0 for k > i: row tile rt needs k < 16 (rt + 1)
+#pragma unroll 4
+            for (int ks = 0; ks < 4 * (rt + 1); ++ks) {
+              const int k = 4 * ks + (lane >> 4);
+              const C sv = s_u[k * UP + ct * 16 + (lane & 15)];
+              const C xv = Xs[(size_t)k * KB + rt * 16 + (lane & 15)];
+              ur[rt] = Mfma<T>::mac(sv.x, xv.x, ur[rt]);
+              ur[rt] = Mfma<T>::mac(-sv.y, xv.y, ur[rt]);
+              ui[rt] = Mfma<T>::mac(sv.y, xv.x, ui[rt]);
+              ui[rt] = Mfma<T>::mac(sv.x, xv.y, ui[rt]);
+            }
+          }
+          __syncthreads();     // S fully read
+#pragma unroll
+          for (int rt = 0; rt < 4; ++rt) {
+            const int own = (NWC == 1) ? 0 : (NWC == 2 ? ((rt == 0 || rt == 3) ? 0 : 1) : rt);
+            if (own != h || !colok) continue;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              const int j = ct * 16 + Mfma<T>::row_of(lane, i), row = rt * 16 + (lane & 15);
+              if (j < gw) {
+                const C u = mk<T>(ur[rt][i], ui[rt][i]);
+                s_u[row * UP + j] = u;
+                A[(size_t)(c0 + j) * lda + K0 + row] = u;
+              }
             }
           }
           __syncthreads();
-#pragma unroll
-          for (int q = 0; q < MAXU; ++q) {
-            const int e = t + q * NT;
-            if (e < NB * gw) {
-              const int i = e / gw, j = e - (e / gw) * gw;
-              s_u[(b * NB + i) * UP + j] = u[q];
-            }
-          }
-          __syncthreads();
-          const int r0 = (b + 1) * NB;
-          for (int e = t; e < (kb - r0) * gw; e += NT) {
-            const int r = r0 + e / gw, j = e - (e / gw) * gw;
-            C s = s_u[r * UP + j];
-            for (int k = 0; k < NB; ++k)
-              s = cfms(s, A[(size_t)(K0 + b * NB + k) * lda + K0 + r], s_u[(b * NB + k) * UP + j]);
-            s_u[r * UP + j] = s;
-          }
-          __syncthreads();
-        }
-        // U12 to memory
-        for (int e = t; e < kb * gw; e += NT) {
-          const int k = e % kb, j = e / kb;
-          A[(size_t)(c0 + j) * lda + K0 + k] = s_u[k * UP + j];
         }
         // A22 -= L21 U12: each wave takes chunks of 32 rows (two 16-row
         // tiles) against the gw columns of the strip
@@ -488,12 +537,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
   if (t == 0) info[blockIdx.x] = myinfo;
 }
 
-// Inverses of the diagonal blocks of L (unit lower) and U (upper) of order
-// BS = 64 (the last block may be shorter; entries outside it are stored as
-// zeros), in FP64 arithmetic, stored as T, row-major: with nblk =
-// ceil(npad / BS), block b of inv(L) at blk_off + b BS^2 and of inv(U) at
-// blk_off + (nblk + b) BS^2.  The per-point allocation is 2 nblk BS^2
-// elements (blk_elems above).
+// Inverses of the diagonal blocks of U (upper) of order BS = 64 (the last
+// block may be shorter; entries outside it are stored as zeros), in FP64
+// arithmetic, stored as T, column-major.  With nblk = ceil(npad / BS), block b
+// of inv(L) is at blk_off + b BS^2 (written by k_lu as it factors) and block b
+// of inv(U) at blk_off + (nblk + b) BS^2; the per-point allocation is
+// 2 nblk BS^2 elements (blk_elems above).
 // grid: x = block index, y = point; 256 threads, four lanes per column: lane
 // q of column c sums the terms k = q (mod 4) of each substitution step and
 // keeps the entries x[k][c], k = q (mod 4), in registers; the step's sum is
@@ -509,7 +558,6 @@ __global__ __launch_bounds__(4 * BS) void k_blkinv(const PointDev *__restrict__ 
   const int bs = min(BS, n - k0), t = threadIdx.x;
   const int nblk = (n + BS - 1) / BS;
   const cx<T> *A = mats + P.mat_off;
-  cx<T> *Li = blks + P.blk_off + (size_t)blockIdx.x * BS * BS;
   cx<T> *Ui = blks + P.blk_off + (size_t)(nblk + blockIdx.x) * BS * BS;
   __shared__ zd s_a[BS * BS];     // row-major copy of the block
   for (int e = t; e < BS * BS; e += 4 * BS) {
@@ -519,28 +567,6 @@ __global__ __launch_bounds__(4 * BS) void k_blkinv(const PointDev *__restrict__ 
   __syncthreads();
   const int c = t >> 2, q = t & 3;
   zd x[BS / 4];                   // x[m] = entry (4 m + q, c) of the inverse being built
-  // inv(L) column c: x_i = delta_ic - sum_{c <= k < i} L_ik x_k, i > c
-#pragma unroll
-  for (int m = 0; m < BS / 4; ++m) x[m] = czero<double>();
-#pragma unroll
-  for (int mi = 0; mi < BS / 4; ++mi) {
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int i = 4 * mi + r;
-      zd s = czero<double>();
-#pragma unroll
-      for (int m = 0; m <= mi; ++m) {
-        const int k = 4 * m + q;
-        if (k < i && k >= c) s = cfma(s, s_a[i * BS + k], x[m]);
-      }
-      s.x += __shfl_xor(s.x, 1); s.y += __shfl_xor(s.y, 1);
-      s.x += __shfl_xor(s.x, 2); s.y += __shfl_xor(s.y, 2);
-      const zd v = (i < c || i >= bs || c >= bs) ? czero<double>() : ((i == c) ? mk(1.0, 0.0) : czero<double>() - s);
-      if (r == q) x[mi] = v;
-    }
-  }
-#pragma unroll
-  for (int m = 0; m < BS / 4; ++m) Li[(4 * m + q) * BS + c] = cvt<T>(x[m]);
   // inv(U) column c: x_i = (delta_ic - sum_{i < k <= c} U_ik x_k) / U_ii, i = c .. 0
 #pragma unroll
   for (int m = 0; m < BS / 4; ++m) x[m] = czero<double>();
@@ -563,5 +589,5 @@ __global__ __launch_bounds__(4 * BS) void k_blkinv(const PointDev *__restrict__ 
     }
   }
 #pragma unroll
-  for (int m = 0; m < BS / 4; ++m) Ui[(4 * m + q) * BS + c] = cvt<T>(x[m]);
+  for (int m = 0; m < BS / 4; ++m) Ui[c * BS + 4 * m + q] = cvt<T>(x[m]);
 }
