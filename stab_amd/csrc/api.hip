@@ -1035,8 +1035,75 @@ static int lead_invit(stab_ctx *c, int npts, const stab_point *pts, int mode, in
     o.info[i] = inf[i] ? STAB_INFO_SINGULAR : (ok[i] ? STAB_INFO_OK : STAB_INFO_NOCONV);
     found += ok[i];
   }
+  // Second pass on the same factors (at most 2 maxit steps): two-vector
+  // iteration (k_invit2) for the
+  // temporal points the first pass missed, and for those it reached only
+  // after 3/4 of its steps (a second eigenvalue nearly as close to the shift,
+  // which may be the wanted one).  Of the two Ritz pairs the wanted one
+  // (leading discrete, or nearest the shift) with residual below 1e-14 is
+  // taken, in column sel[i] of the point's vector block, unless the other
+  // pair, not converged, would rank above it.
+  std::vector<int> sel(npts, 0), sub;
+  for (int i = 0; i < npts; ++i)
+    if (!inf[i] && pts[i].itype == 1 && (!ok[i] || 4 * it[i] >= 3 * (maxit + 1))) sub.push_back(i);
+  const size_t lds2 = lds + sizeof(zd) * 6 * (size_t)B.npadmax;
+  float ms2 = 0.f;
+  if (!sub.empty() && lds2 <= 160 * 1024) {
+    const int ns = (int)sub.size();
+    zd *ev2 = B.eval + npts;
+    double *rs2 = B.resid + npts, *ff2 = B.ff + npts;    // 2 npts entries each (KS_NEVMAX npts allocated)
+    HIPCHECK(c, hipMemcpyAsync(B.sel, sub.data(), sizeof(int) * ns, hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipEventRecord(c->ev[5], c->stream));
+    if (f32)
+      hipLaunchKernelGGL((k_invit2<float, 16, 256>), dim3(ns), dim3(256), lds2, c->stream, B.pts, B.sel,
+                         (const zf *)B.mats, B.pivs, (const zf *)B.blks, B.scales, B.coef, B.ops, B.norms, 2 * maxit, tol,
+                         ev2, B.evec, rs2, ff2, B.iters, B.npadmax, B.nymax);
+    else
+      hipLaunchKernelGGL((k_invit2<double, 16, 256>), dim3(ns), dim3(256), lds2, c->stream, B.pts, B.sel,
+                         (const zd *)B.mats, B.pivs, (const zd *)B.blks, B.scales, B.coef, B.ops, B.norms, 2 * maxit, tol,
+                         ev2, B.evec, rs2, ff2, B.iters, B.npadmax, B.nymax);
+    HIPCHECK(c, hipGetLastError());
+    HIPCHECK(c, hipEventRecord(c->ev[6], c->stream));
+    std::vector<zd> e2(2 * (size_t)npts);
+    std::vector<double> r2(2 * (size_t)npts), f2(2 * (size_t)npts);
+    HIPCHECK(c, hipMemcpyAsync(e2.data(), ev2, sizeof(zd) * 2 * npts, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(r2.data(), rs2, sizeof(double) * 2 * npts, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(f2.data(), ff2, sizeof(double) * 2 * npts, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(it.data(), B.iters, sizeof(int) * npts, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    ms2 = elapsed(c->ev[5], c->ev[6]);
+    for (int i : sub) {
+      const zd s0 = mk(pts[i].shift.re, pts[i].shift.im);
+      // rank of a pair: distance to the shift (nearest) or the leading key
+      auto better = [&](zd a, zd b) { return nearest ? cabs(a - s0) < cabs(b - s0) : lead_key(pts[i], a) > lead_key(pts[i], b); };
+      int best = -1;
+      for (int k = 0; k < 2; ++k) {
+        const size_t e = 2 * (size_t)i + k;
+        if (!(r2[e] < 1e-14)) continue;
+        if (!nearest && !lead_discrete(pts[i], e2[e], f2[e])) continue;
+        if (best < 0 || better(e2[e], e2[2 * (size_t)i + best])) best = k;
+      }
+      if (best >= 0) {
+        const size_t o2 = 2 * (size_t)i + (1 - best);
+        if (!(r2[o2] < 1e-14) && better(e2[o2], e2[2 * (size_t)i + best])) best = -1;
+      }
+      if (o.iters) o.iters[i] = it[i];
+      if (best < 0) {          // a first-pass answer taken back to the second pass stays rejected
+        if (ok[i]) { ok[i] = 0; --found; o.info[i] = STAB_INFO_NOCONV; }
+        continue;
+      }
+      const size_t e = 2 * (size_t)i + best;
+      if (!ok[i]) ++found;
+      ok[i] = 1;
+      sel[i] = best;
+      o.eval[i].re = e2[e].x;
+      o.eval[i].im = e2[e].y;
+      if (o.resid) o.resid[i] = r2[e];
+      if (o.ff) o.ff[i] = f2[e];
+      o.info[i] = STAB_INFO_OK;
+    }
+  }
   if (o.evec) {
-    std::vector<int> sel(npts, 0);
     if ((rc = gather_lead_vecs(c, npts, B, sel, o.evec)) != STAB_OK) return rc;
   }
   if (stats) {
@@ -1045,8 +1112,8 @@ static int lead_invit(stab_ctx *c, int npts, const stab_point *pts, int mode, in
     ri.ms_assemble = elapsed(c->ev[1], c->ev[2]);
     ri.ms_lu = elapsed(c->ev[2], c->ev[3]);
     ri.ms_lu_kernel = elapsed(c->ev[2], c->ev[7]);
-    ri.ms_eig = elapsed(c->ev[3], c->ev[4]);
-    ri.ms_total = elapsed(c->ev[0], c->ev[4]);
+    ri.ms_eig = elapsed(c->ev[3], c->ev[4]) + ms2;
+    ri.ms_total = elapsed(c->ev[0], c->ev[4]) + ms2;
     ri.ms_arnoldi = ri.ms_eig;
     ri.cycles = 1;
     ri.nconv = found;

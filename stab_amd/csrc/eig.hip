@@ -894,7 +894,7 @@ __global__ __launch_bounds__(NT) void k_solve_one(const PointDev *__restrict__ p
 // vector normalised by its first entry of largest modulus (z = [phi;
 // lambda phi] for spatial points, at out_off), the residual, the farfield
 // ratio, the number of solves and a convergence flag (0, or 2 when the
-// residual stayed above 1e-13).
+// residual stayed above 1e-14).
 // ---------------------------------------------------------------------------
 template <typename T, int NB, int NT>
 __global__ __launch_bounds__(NT) void k_invit(const PointDev *__restrict__ pts, const cx<T> *__restrict__ mats,
@@ -1041,8 +1041,196 @@ __global__ __launch_bounds__(NT) void k_invit(const PointDev *__restrict__ pts, 
     resid[blockIdx.x] = rel;
     ffo[blockIdx.x] = ff;
     iters[blockIdx.x] = nsolve;
-    flag[blockIdx.x] = (rel < 1e-13) ? 0 : 2;
+    flag[blockIdx.x] = (rel < 1e-14) ? 0 : 2;
   }
+}
+
+// Principal square root of a complex number.
+__device__ inline zd zsqrt(zd z) {
+  const double r = cabs(z);
+  const double sr = sqrt(0.5 * (r + fabs(z.x)));
+  if (sr == 0.0) return czero<double>();
+  if (z.x >= 0) return mk(sr, z.y / (2.0 * sr));
+  return mk(fabs(z.y) / (2.0 * sr), z.y >= 0 ? sr : -sr);
+}
+
+// ---------------------------------------------------------------------------
+// Two-vector residual inverse iteration with a 2 x 2 Rayleigh-Ritz step, for
+// temporal points whose single-vector iteration (k_invit) missed.  There the
+// two eigenvalues nearest the shift are nearly equidistant from it, so one
+// vector contracts towards neither (or settles on the nearer one when the
+// other is wanted); a basis X = [x1, x2] converges to both at the rate
+// max |lambda_{1,2} - sigma| / |lambda_3 - sigma| per step:
+//   X orthonormal (FP64); projected pencil (X^H A X, X^H B X); its two
+//   eigenpairs (theta_k, y_k) from det(A~ - theta B~) = 0; Ritz vectors
+//   u_k = X y_k with FP64 residuals r_k = A u_k - theta_k B u_k; new basis
+//   x_k = u_k - L(sigma)^-1 r_k.
+// Starts from the vector k_invit left in column 0 of the point's block (x1)
+// and x2 = L(sigma)^-1 B x1, on the factors k_invit used.  Stops when both
+// relative residuals are below tol, when their maximum has not halved for three
+// steps while below 1e-12, or after maxit steps.
+// grid: one work-group per listed point p = sub[blockIdx.x].  Outputs, k = 0, 1:
+// eval2[2 p + k], resid2[2 p + k], ff2[2 p + k] and the Ritz vector
+// (getevec normalisation) in column k of p's block; iters[p] += solves.
+// LDS: the full operator layout followed by six npadmax vectors.
+// ---------------------------------------------------------------------------
+template <typename T, int NB, int NT>
+__global__ __launch_bounds__(NT) void k_invit2(const PointDev *__restrict__ pts, const int *__restrict__ sub,
+                                               const cx<T> *__restrict__ mats, const int *__restrict__ pivs,
+                                               const cx<T> *__restrict__ blks, const double *__restrict__ scales,
+                                               const zd *__restrict__ coef, const double *__restrict__ ops,
+                                               const double *__restrict__ norms, int maxit, double tol,
+                                               zd *__restrict__ eval2, zd *__restrict__ evec,
+                                               double *__restrict__ resid2, double *__restrict__ ff2,
+                                               int *__restrict__ iters, int npadmax, int nymax) {
+  extern __shared__ zd lds[];
+  const int p = sub[blockIdx.x];
+  const PointDev &P = pts[p];
+  OpLds L(lds, P, npadmax, nymax, 1, npadmax);
+  zd *x1 = L.X2 + 5 * nymax;
+  zd *x2 = x1 + npadmax, *a1 = x2 + npadmax, *b1 = a1 + npadmax, *a2 = b1 + npadmax, *b2 = a2 + npadmax;
+  const int t = threadIdx.x, n = P.n, nv = P.nvec;
+  const cx<T> *A = mats + P.mat_off;
+  const int *ipiv = pivs + P.piv_off;
+  const cx<T> *Lb = blks + P.blk_off, *Ub = nullptr;
+  const double *dr = scales + P.scale_off, *dc = dr + P.npad;
+  const double nA = sqrt(norms[p * 4 + 0]), nB = sqrt(norms[p * 4 + 1]);
+  const zd sg = P.shift;
+  zd *out = evec + P.out_off;
+  __shared__ double s_mx[NT];
+  __shared__ int s_ix[NT];
+
+  for (int r = t; r < n; r += NT) x1[r] = out[r];
+  __syncthreads();
+  pencil_apply<NT>(P, coef, ops, FORM_BM, sg, x1, L.rhs, L.X1, L.X2, L.red);
+  shifted_solve<T, NB, NT>(P, A, ipiv, Lb, Ub, dr, dc, coef, ops, 0, L);
+  for (int r = t; r < n; r += NT) x2[r] = L.y[r];
+  __syncthreads();
+  int nsolve = 1;
+  zd th[2] = {sg, sg};
+  double rel[2] = {1e300, 1e300}, best = 1e300;
+  int stall = 0;
+  for (int it = 0;; ++it) {
+    // orthonormal basis
+    {
+      const double n1 = sqrt(block_norm2<NT>(x1, n, L.red));
+      const double i1 = (n1 > 0.0) ? 1.0 / n1 : 1.0;
+      zd d[1] = {czero<double>()};
+      for (int r = t; r < n; r += NT) {
+        x1[r] = x1[r] * i1;
+        d[0] = cfmac(d[0], x1[r], x2[r]);
+      }
+      block_reduce<NT, 1>(d, L.red);
+      for (int r = t; r < n; r += NT) x2[r] = x2[r] - d[0] * x1[r];
+      __syncthreads();
+      const double n2 = sqrt(block_norm2<NT>(x2, n, L.red));
+      const double i2 = (n2 > 0.0) ? 1.0 / n2 : 1.0;
+      for (int r = t; r < n; r += NT) x2[r] = x2[r] * i2;
+      __syncthreads();
+    }
+    pencil_apply<NT>(P, coef, ops, FORM_AM, sg, x1, a1, L.X1, L.X2, L.red);
+    pencil_apply<NT>(P, coef, ops, FORM_BM, sg, x1, b1, L.X1, L.X2, L.red);
+    pencil_apply<NT>(P, coef, ops, FORM_AM, sg, x2, a2, L.X1, L.X2, L.red);
+    pencil_apply<NT>(P, coef, ops, FORM_BM, sg, x2, b2, L.X1, L.X2, L.red);
+    // projected pencil: pa[i][j] = x_i^H a_j, pb[i][j] = x_i^H b_j
+    zd pr[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) pr[k] = czero<double>();
+    for (int r = t; r < n; r += NT) {
+      const zd u = x1[r], v = x2[r];
+      pr[0] = cfmac(pr[0], u, a1[r]); pr[1] = cfmac(pr[1], u, a2[r]);
+      pr[2] = cfmac(pr[2], v, a1[r]); pr[3] = cfmac(pr[3], v, a2[r]);
+      pr[4] = cfmac(pr[4], u, b1[r]); pr[5] = cfmac(pr[5], u, b2[r]);
+      pr[6] = cfmac(pr[6], v, b1[r]); pr[7] = cfmac(pr[7], v, b2[r]);
+    }
+    block_reduce<NT, 8>(pr, L.red);
+    const zd A00 = pr[0], A01 = pr[1], A10 = pr[2], A11 = pr[3];
+    const zd B00 = pr[4], B01 = pr[5], B10 = pr[6], B11 = pr[7];
+    // det(A~ - theta B~) = qa theta^2 + qb theta + qc
+    const zd qa = B00 * B11 - B01 * B10;
+    const zd qb = czero<double>() - (A00 * B11 + A11 * B00 - A01 * B10 - A10 * B01);
+    const zd qc = A00 * A11 - A01 * A10;
+    if (!(cabs(qa) > 0.0)) break;            // B~ singular: leave rel at its last value
+    {
+      const zd sq = zsqrt(qb * qb - qa * qc * 4.0);
+      // q = -(qb + s sq) / 2 with the sign that avoids cancellation
+      const zd s1 = qb + sq, s2 = qb - sq;
+      const zd q = (cabs(s1) >= cabs(s2) ? s1 : s2) * -0.5;
+      th[0] = cdiv(q, qa);
+      th[1] = (cabs(q) > 0.0) ? cdiv(qc, q) : th[0];
+    }
+    // 2-vectors y_k with (A~ - theta_k B~) y_k = 0, unit norm
+    zd y[2][2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const zd m00 = A00 - th[k] * B00, m01 = A01 - th[k] * B01, m10 = A10 - th[k] * B10, m11 = A11 - th[k] * B11;
+      zd y0 = czero<double>() - m01, y1 = m00;
+      if (abs2(m11) + abs2(m10) > abs2(m00) + abs2(m01)) { y0 = m11; y1 = czero<double>() - m10; }
+      double ny = sqrt(abs2(y0) + abs2(y1));
+      if (!(ny > 0.0)) { y0 = mk(k == 0 ? 1.0 : 0.0, 0.0); y1 = mk(k == 0 ? 0.0 : 1.0, 0.0); ny = 1.0; }
+      y[k][0] = y0 * (1.0 / ny);
+      y[k][1] = y1 * (1.0 / ny);
+    }
+    // Ritz vectors into x1, x2 and residuals into a1, a2 (element-wise in place)
+    zd nr[4] = {czero<double>(), czero<double>(), czero<double>(), czero<double>()};
+    for (int r = t; r < n; r += NT) {
+      const zd u = x1[r], v = x2[r];
+      const zd u0 = y[0][0] * u + y[0][1] * v, u1 = y[1][0] * u + y[1][1] * v;
+      const zd r0 = (y[0][0] * a1[r] + y[0][1] * a2[r]) - th[0] * (y[0][0] * b1[r] + y[0][1] * b2[r]);
+      const zd r1 = (y[1][0] * a1[r] + y[1][1] * a2[r]) - th[1] * (y[1][0] * b1[r] + y[1][1] * b2[r]);
+      x1[r] = u0; x2[r] = u1; a1[r] = r0; a2[r] = r1;
+      nr[0].x += abs2(u0); nr[1].x += abs2(u1); nr[2].x += abs2(r0); nr[3].x += abs2(r1);
+    }
+    block_reduce<NT, 4>(nr, L.red);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const double un = sqrt(nr[k].x), den = (nA + cabs(th[k]) * nB) * un;
+      rel[k] = (den > 0.0) ? sqrt(nr[2 + k].x) / den : 1e300;
+    }
+    const double mx = fmax(rel[0], rel[1]);
+    if (mx < 0.5 * best) { best = mx; stall = 0; } else if (mx < 1e-12) { ++stall; }
+    if (mx < tol || it >= maxit || stall >= 3) break;
+    // corrections x_k = u_k - L(sigma)^-1 r_k
+    for (int r = t; r < n; r += NT) L.rhs[r] = a1[r];
+    __syncthreads();
+    shifted_solve<T, NB, NT>(P, A, ipiv, Lb, Ub, dr, dc, coef, ops, 0, L);
+    for (int r = t; r < n; r += NT) { x1[r] = x1[r] - L.y[r]; L.rhs[r] = a2[r]; }
+    __syncthreads();
+    shifted_solve<T, NB, NT>(P, A, ipiv, Lb, Ub, dr, dc, coef, ops, 0, L);
+    for (int r = t; r < n; r += NT) x2[r] = x2[r] - L.y[r];
+    __syncthreads();
+    nsolve += 2;
+  }
+  // ---- both Ritz vectors, getevec normalisation (first entry of largest
+  // modulus set to 1), farfield ratio, out to columns 0 and 1
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    zd *x = (k == 0) ? x1 : x2;
+    double bm = -1.0;
+    int bi = 0x7fffffff;
+    for (int r = t; r < n; r += NT) { const double a = abs2(x[r]); if (a > bm) { bm = a; bi = r; } }
+    s_mx[t] = bm; s_ix[t] = bi;
+    __syncthreads();
+    if (t == 0) {
+      for (int j = 1; j < NT; ++j)
+        if (s_mx[j] > bm || (s_mx[j] == bm && s_ix[j] < bi)) { bm = s_mx[j]; bi = s_ix[j]; }
+      s_ix[0] = bi;
+    }
+    __syncthreads();
+    const zd scale = x[s_ix[0]];
+    __syncthreads();
+    if (abs2(scale) > 0.0)
+      for (int r = t; r < n; r += NT) x[r] = cdiv(x[r], scale);
+    __syncthreads();
+    const double ff = farfield_ratio<NT>(P, x, L.red);
+    for (int r = t; r < n; r += NT) out[(size_t)k * nv + r] = x[r];
+    if (t == 0) {
+      eval2[2 * p + k] = th[k];
+      resid2[2 * p + k] = rel[k];
+      ff2[2 * p + k] = ff;
+    }
+  }
+  if (t == 0) iters[p] += nsolve;
 }
 
 // Copy column sel[p] of each point's eigenvector block (P.out_off, nvec rows,
