@@ -1042,11 +1042,11 @@ static int lead_invit(stab_ctx *c, int npts, const stab_point *pts, int mode, in
   // which may be the wanted one).  Of the two Ritz pairs the wanted one
   // (leading discrete, or nearest the shift) with residual below 1e-14 is
   // taken, in column sel[i] of the point's vector block, unless the other
-  // pair, not converged, would rank above it.
+  // pair, not converged, could still outrank it (lead2_competes).
   std::vector<int> sel(npts, 0), sub;
   for (int i = 0; i < npts; ++i)
     if (!inf[i] && pts[i].itype == 1 && (!ok[i] || 4 * it[i] >= 3 * (maxit + 1))) sub.push_back(i);
-  const size_t lds2 = lds + sizeof(zd) * 6 * (size_t)B.npadmax;
+  const size_t lds2 = lds + sizeof(zd) * (6 * (size_t)B.npadmax + 2 * 256);
   float ms2 = 0.f;
   if (!sub.empty() && lds2 <= 160 * 1024) {
     const int ns = (int)sub.size();
@@ -1083,9 +1083,11 @@ static int lead_invit(stab_ctx *c, int npts, const stab_point *pts, int mode, in
         if (!nearest && !lead_discrete(pts[i], e2[e], f2[e])) continue;
         if (best < 0 || better(e2[e], e2[2 * (size_t)i + best])) best = k;
       }
-      if (best >= 0) {
+      if (best >= 0) {       // the other pair, not converged, must not outrank it
         const size_t o2 = 2 * (size_t)i + (1 - best);
-        if (!(r2[o2] < 1e-14) && better(e2[o2], e2[2 * (size_t)i + best])) best = -1;
+        if (!(r2[o2] < 1e-14) && lead2_competes(nearest ? 1 : 0, pts[i].alpha.re, B.h[i].shift,
+                                                e2[2 * (size_t)i + best], e2[o2], r2[o2], f2[o2]))
+          best = -1;
       }
       if (o.iters) o.iters[i] = it[i];
       if (best < 0) {          // a first-pass answer taken back to the second pass stays rejected

@@ -65,37 +65,51 @@ template <> struct Row16<double> {
   __device__ static zd get(const V &v, int) { return mk(v.x, v.y); }
 };
 
-// tmp[0..bs) <- inv(D) y[k0..k0+bs) with D the diagonal block at inv (column
-// major, entries beyond bs zero), tmp[bs..64) <- 0; y[k0..k0+bs) <- the same.
-// Branch-free: indexes past bs are clamped to bs - 1 and meet zero entries.
-// Two barriers.
-template <typename T, int NT>
-__device__ inline void solve_diag(const cx<T> *__restrict__ inv, int k0, int bs, zd *y, zd *tmp) {
+// For each right-hand side r < NR: tmp[r NT + 0..bs) <- inv(D) y_r[k0..k0+bs)
+// with D the diagonal block at inv (column major, entries beyond bs zero),
+// tmp[r NT + bs..64) <- 0, and y_r[k0..k0+bs) <- the same.  The inverse is
+// loaded once for all right-hand sides.  Branch-free: indexes past bs are
+// clamped to bs - 1 and meet zero entries.  Two barriers.
+template <typename T, int NT, int NR>
+__device__ inline void solve_diag(const cx<T> *__restrict__ inv, int k0, int bs, zd *const (&y)[NR], zd *tmp) {
   constexpr int BS = SOLVE_BS;
   const int t = threadIdx.x, i = t & (BS - 1), q = t >> 6;
   const cx<T> *col = inv + (size_t)(q * 16) * BS + i;
   cx<T> v[16];
 #pragma unroll
   for (int j = 0; j < 16; ++j) v[j] = col[j * BS];
-  zd s = czero<double>();
+  zd s[NR];
 #pragma unroll
-  for (int j = 0; j < 16; ++j) s = cfma(s, cvt<double>(v[j]), y[k0 + min(q * 16 + j, bs - 1)]);
-  tmp[t] = s;
+  for (int r = 0; r < NR; ++r) s[r] = czero<double>();
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const zd vj = cvt<double>(v[j]);
+    const int kk = k0 + min(q * 16 + j, bs - 1);
+#pragma unroll
+    for (int r = 0; r < NR; ++r) s[r] = cfma(s[r], vj, y[r][kk]);
+  }
+#pragma unroll
+  for (int r = 0; r < NR; ++r) tmp[r * NT + t] = s[r];
   __syncthreads();
   if (t < BS) {
-    const zd r = tmp[t] + tmp[BS + t] + tmp[2 * BS + t] + tmp[3 * BS + t];
-    tmp[t] = (t < bs) ? r : czero<double>();
-    if (t < bs) y[k0 + t] = r;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      zd *tr = tmp + r * NT;
+      const zd v = tr[t] + tr[BS + t] + tr[2 * BS + t] + tr[3 * BS + t];
+      tr[t] = (t < bs) ? v : czero<double>();
+      if (t < bs) y[r][k0 + t] = v;
+    }
   }
   __syncthreads();
 }
 
-// y[r0..r1) -= A[r0..r1, k0..k0+bs) tmp[0..bs); r0, r1 multiples of E.
+// y_r[r0..r1) -= A[r0..r1, k0..k0+bs) tmp[r NT + 0..bs) for r < NR (the matrix
+// is loaded once for all right-hand sides); r0, r1 multiples of E.
 // Branch-free: column indexes past bs are clamped to bs - 1 (tmp is zero
 // there) and rows past r1 to the last row group (not written back).
-template <typename T, int NT>
-__device__ inline void solve_update(const cx<T> *__restrict__ A, int lda, int k0, int bs, int r0, int r1, zd *y,
-                                    const zd *tmp) {
+template <typename T, int NT, int NR>
+__device__ inline void solve_update(const cx<T> *__restrict__ A, int lda, int k0, int bs, int r0, int r1,
+                                    zd *const (&y)[NR], const zd *tmp) {
   typedef Row16<T> R;
   typedef typename R::V V;
   constexpr int E = R::E, G = NT / 4;        // G row groups of E rows per pass
@@ -107,61 +121,82 @@ __device__ inline void solve_update(const cx<T> *__restrict__ A, int lda, int k0
   for (int j = 0; j < 16; ++j) coff[j] = (k0 + min(q * 16 + j, bs - 1)) * ldv;
 #pragma unroll 1
   for (int rb = r0; rb < r1; rb += G * E) {
-    const int r = rb + g * E, rl = min(r, r1 - E);
+    const int row = rb + g * E, rl = min(row, r1 - E);
     V v[16];
 #pragma unroll
     for (int j = 0; j < 16; ++j) v[j] = base[coff[j] + rl / E];
-    zd a[E];
+    zd a[NR][E];
 #pragma unroll
-    for (int e = 0; e < E; ++e) a[e] = czero<double>();
+    for (int r = 0; r < NR; ++r)
+#pragma unroll
+      for (int e = 0; e < E; ++e) a[r][e] = czero<double>();
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
-      const zd mj = tmp[q * 16 + j];
 #pragma unroll
-      for (int e = 0; e < E; ++e) a[e] = cfma(a[e], R::get(v[j], e), mj);
+      for (int r = 0; r < NR; ++r) {
+        const zd mj = tmp[r * NT + q * 16 + j];
+#pragma unroll
+        for (int e = 0; e < E; ++e) a[r][e] = cfma(a[r][e], R::get(v[j], e), mj);
+      }
     }
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      a[e].x += __shfl_xor(a[e].x, 1); a[e].y += __shfl_xor(a[e].y, 1);
-      a[e].x += __shfl_xor(a[e].x, 2); a[e].y += __shfl_xor(a[e].y, 2);
-    }
-    if (r < r1 && q == 0) {
+    for (int r = 0; r < NR; ++r)
 #pragma unroll
-      for (int e = 0; e < E; ++e) y[r + e] = y[r + e] - a[e];
+      for (int e = 0; e < E; ++e) {
+        a[r][e].x += __shfl_xor(a[r][e].x, 1); a[r][e].y += __shfl_xor(a[r][e].y, 1);
+        a[r][e].x += __shfl_xor(a[r][e].x, 2); a[r][e].y += __shfl_xor(a[r][e].y, 2);
+      }
+    if (row < r1 && q == 0) {
+#pragma unroll
+      for (int r = 0; r < NR; ++r)
+#pragma unroll
+        for (int e = 0; e < E; ++e) y[r][row + e] = y[r][row + e] - a[r][e];
     }
   }
 }
 
-template <typename T, int NT>
-__device__ __forceinline__ void lu_solve(const PointDev &P, const cx<T> *__restrict__ A, const int *__restrict__ ipiv,
-                         const cx<T> *__restrict__ blk, zd *y, zd *tmp, zd *work) {
+// NR right-hand sides y[r] (LDS, npad) solved together; tmp holds NR NT
+// entries, work npad.
+template <typename T, int NT, int NR>
+__device__ __forceinline__ void lu_solve_n(const PointDev &P, const cx<T> *__restrict__ A, const int *__restrict__ ipiv,
+                                           const cx<T> *__restrict__ blk, zd *const (&y)[NR], zd *tmp, zd *work) {
   static_assert(NT == 4 * SOLVE_BS, "four lanes per row of a 64-row block");
   constexpr int BS = SOLVE_BS;
   const int n = P.npad, lda = P.npad, t = threadIdx.x;
   const int *perm = ipiv + n;
   const int nblk = (n + BS - 1) / BS;
   const cx<T> *Li = blk, *Ui = blk + (size_t)nblk * BS * BS;   // layout of k_blkinv
-  // y <- P y (one gather through `work`)
-  for (int r = t; r < n; r += NT) work[r] = y[perm[r]];
-  __syncthreads();
-  for (int r = t; r < n; r += NT) y[r] = work[r];
-  __syncthreads();
+  // y <- P y (one gather through `work` per right-hand side)
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    for (int i = t; i < n; i += NT) work[i] = y[r][perm[i]];
+    __syncthreads();
+    for (int i = t; i < n; i += NT) y[r][i] = work[i];
+    __syncthreads();
+  }
   for (int b = 0; b < nblk; ++b) {
     const int k0 = b * BS, bs = min(BS, n - k0);
-    solve_diag<T, NT>(Li + (size_t)b * BS * BS, k0, bs, y, tmp);
+    solve_diag<T, NT, NR>(Li + (size_t)b * BS * BS, k0, bs, y, tmp);
     if (k0 + bs < n) {
-      solve_update<T, NT>(A, lda, k0, bs, k0 + bs, n, y, tmp);
+      solve_update<T, NT, NR>(A, lda, k0, bs, k0 + bs, n, y, tmp);
       __syncthreads();
     }
   }
   for (int b = nblk - 1; b >= 0; --b) {
     const int k0 = b * BS, bs = min(BS, n - k0);
-    solve_diag<T, NT>(Ui + (size_t)b * BS * BS, k0, bs, y, tmp);
+    solve_diag<T, NT, NR>(Ui + (size_t)b * BS * BS, k0, bs, y, tmp);
     if (k0 > 0) {
-      solve_update<T, NT>(A, lda, k0, bs, 0, k0, y, tmp);
+      solve_update<T, NT, NR>(A, lda, k0, bs, 0, k0, y, tmp);
       __syncthreads();
     }
   }
+}
+
+template <typename T, int NT>
+__device__ __forceinline__ void lu_solve(const PointDev &P, const cx<T> *__restrict__ A, const int *__restrict__ ipiv,
+                                         const cx<T> *__restrict__ blk, zd *y, zd *tmp, zd *work) {
+  zd *const ys[1] = {y};
+  lu_solve_n<T, NT, 1>(P, A, ipiv, blk, ys, tmp, work);
 }
 
 // Block-wide reduction of NV complex partial sums; result broadcast to all.
@@ -1066,14 +1101,37 @@ __device__ inline zd zsqrt(zd z) {
 //   u_k = X y_k with FP64 residuals r_k = A u_k - theta_k B u_k; new basis
 //   x_k = u_k - L(sigma)^-1 r_k.
 // Starts from the vector k_invit left in column 0 of the point's block (x1)
-// and x2 = L(sigma)^-1 B x1, on the factors k_invit used.  Stops when both
-// relative residuals are below tol, when their maximum has not halved for three
-// steps while below 1e-12, or after maxit steps.
+// and x2 = L(sigma)^-1 B x1, on the factors k_invit used.  A pair has
+// converged when its relative residual is below tol or has not halved for
+// three steps while below 1e-12; both pairs are iterated to convergence, at
+// most maxit steps (the host then takes the wanted one, see lead2_competes).
 // grid: one work-group per listed point p = sub[blockIdx.x].  Outputs, k = 0, 1:
 // eval2[2 p + k], resid2[2 p + k], ff2[2 p + k] and the Ritz vector
 // (getevec normalisation) in column k of p's block; iters[p] += solves.
-// LDS: the full operator layout followed by six npadmax vectors.
+// LDS: the full operator layout followed by six npadmax vectors and 2 NT
+// entries.
 // ---------------------------------------------------------------------------
+// The discrete-mode test of the leading-mode selection for temporal points:
+// farfield ratio below STAB_FARFIELD_MAX and phase speed off the continuum.
+__host__ __device__ inline bool lead2_discrete(double alpha_r, zd ev, double ff) {
+  if (!(ff < STAB_FARFIELD_MAX)) return false;
+  if (alpha_r == 0.0) return true;
+  return fabs(ev.x / alpha_r - 1.0) > STAB_CONTINUUM_BAND;
+}
+
+// Whether pair o (residual rel_o, farfield ratio ff_o), not converged, could
+// still outrank the converged pair w.  Its eigenvalue error is taken as
+// 1e4 rel_o max(|theta_o|, 1e-2) (a relative condition number of 1e4, typical
+// of the discrete modes of these pencils); its farfield ratio is trusted once
+// rel_o < 1e-8.
+__host__ __device__ inline bool lead2_competes(int want, double alpha_r, zd sigma, zd th_w, zd th_o, double rel_o,
+                                               double ff_o) {
+  const double err = 1e4 * rel_o * fmax(cabs(th_o), 1e-2);
+  if (want == 1) return cabs(th_o - sigma) < cabs(th_w - sigma) + err;
+  const bool maybe_discrete = !(rel_o < 1e-8) || lead2_discrete(alpha_r, th_o, ff_o);
+  return maybe_discrete && th_o.y > th_w.y - err;
+}
+
 template <typename T, int NB, int NT>
 __global__ __launch_bounds__(NT) void k_invit2(const PointDev *__restrict__ pts, const int *__restrict__ sub,
                                                const cx<T> *__restrict__ mats, const int *__restrict__ pivs,
@@ -1089,6 +1147,7 @@ __global__ __launch_bounds__(NT) void k_invit2(const PointDev *__restrict__ pts,
   OpLds L(lds, P, npadmax, nymax, 1, npadmax);
   zd *x1 = L.X2 + 5 * nymax;
   zd *x2 = x1 + npadmax, *a1 = x2 + npadmax, *b1 = a1 + npadmax, *a2 = b1 + npadmax, *b2 = a2 + npadmax;
+  zd *tmp2 = b2 + npadmax;                     // 2 NT entries for the two-column solve
   const int t = threadIdx.x, n = P.n, nv = P.nvec;
   const cx<T> *A = mats + P.mat_off;
   const int *ipiv = pivs + P.piv_off;
@@ -1108,8 +1167,8 @@ __global__ __launch_bounds__(NT) void k_invit2(const PointDev *__restrict__ pts,
   __syncthreads();
   int nsolve = 1;
   zd th[2] = {sg, sg};
-  double rel[2] = {1e300, 1e300}, best = 1e300;
-  int stall = 0;
+  double rel[2] = {1e300, 1e300}, best[2] = {1e300, 1e300};
+  int stall[2] = {0, 0};
   for (int it = 0;; ++it) {
     // orthonormal basis
     {
@@ -1156,8 +1215,12 @@ __global__ __launch_bounds__(NT) void k_invit2(const PointDev *__restrict__ pts,
       // q = -(qb + s sq) / 2 with the sign that avoids cancellation
       const zd s1 = qb + sq, s2 = qb - sq;
       const zd q = (cabs(s1) >= cabs(s2) ? s1 : s2) * -0.5;
-      th[0] = cdiv(q, qa);
-      th[1] = (cabs(q) > 0.0) ? cdiv(qc, q) : th[0];
+      const zd t0 = cdiv(q, qa), t1 = (cabs(q) > 0.0) ? cdiv(qc, q) : t0;
+      // keep each pair's index from the previous step (its convergence state
+      // is tracked per index)
+      const bool swap = it > 0 && cabs(t0 - th[1]) + cabs(t1 - th[0]) < cabs(t0 - th[0]) + cabs(t1 - th[1]);
+      th[0] = swap ? t1 : t0;
+      th[1] = swap ? t0 : t1;
     }
     // 2-vectors y_k with (A~ - theta_k B~) y_k = 0, unit norm
     zd y[2][2];
@@ -1187,17 +1250,28 @@ __global__ __launch_bounds__(NT) void k_invit2(const PointDev *__restrict__ pts,
       const double un = sqrt(nr[k].x), den = (nA + cabs(th[k]) * nB) * un;
       rel[k] = (den > 0.0) ? sqrt(nr[2 + k].x) / den : 1e300;
     }
-    const double mx = fmax(rel[0], rel[1]);
-    if (mx < 0.5 * best) { best = mx; stall = 0; } else if (mx < 1e-12) { ++stall; }
-    if (mx < tol || it >= maxit || stall >= 3) break;
-    // corrections x_k = u_k - L(sigma)^-1 r_k
-    for (int r = t; r < n; r += NT) L.rhs[r] = a1[r];
+    bool conv[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      if (rel[k] < 0.5 * best[k]) { best[k] = rel[k]; stall[k] = 0; } else if (rel[k] < 1e-12) { ++stall[k]; }
+      conv[k] = rel[k] < tol || stall[k] >= 3;
+    }
+    if ((conv[0] && conv[1]) || it >= maxit) break;
+    // corrections x_k = u_k - L(sigma)^-1 r_k: both solves in one pass over
+    // the factors, in place in a1, a2 (scaled rows, zero padding)
+    for (int r = t; r < P.npad; r += NT) {
+      a1[r] = (r < n) ? a1[r] * dr[r] : czero<double>();
+      a2[r] = (r < n) ? a2[r] * dr[r] : czero<double>();
+    }
     __syncthreads();
-    shifted_solve<T, NB, NT>(P, A, ipiv, Lb, Ub, dr, dc, coef, ops, 0, L);
-    for (int r = t; r < n; r += NT) { x1[r] = x1[r] - L.y[r]; L.rhs[r] = a2[r]; }
-    __syncthreads();
-    shifted_solve<T, NB, NT>(P, A, ipiv, Lb, Ub, dr, dc, coef, ops, 0, L);
-    for (int r = t; r < n; r += NT) x2[r] = x2[r] - L.y[r];
+    {
+      zd *const ys[2] = {a1, a2};
+      lu_solve_n<T, NT, 2>(P, A, ipiv, Lb, ys, tmp2, L.w2);
+    }
+    for (int r = t; r < n; r += NT) {
+      x1[r] = x1[r] - a1[r] * dc[r];
+      x2[r] = x2[r] - a2[r] * dc[r];
+    }
     __syncthreads();
     nsolve += 2;
   }

@@ -117,10 +117,13 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
         for (int c = 0; c < NB; ++c)
           Pn[j][c] = (r >= k0 && r < n) ? A[(size_t)(k0 + c) * lda + r] : czero<T>();
       }
-      // ---- factor the panel.  The column loop is unrolled, so every register
-      // access below has compile-time indices (Pn stays in VGPRs) and the
-      // rank-1 updates touch only the columns right of the pivot.
-#pragma unroll
+      // ---- factor the panel.  Every register access below has compile-time
+      // column indices (with predicates when the loop is not unrolled), so Pn
+      // stays in VGPRs.  FP32 unrolls the column loop, so the rank-1 updates
+      // touch only the columns right of the pivot; FP64 (more registers per
+      // entry) keeps it rolled.
+      constexpr int PANEL_UNROLL = sizeof(T) == 4 ? NB : 1;
+#pragma unroll PANEL_UNROLL
       for (int c = 0; c < NB; ++c) {
         const int col = k0 + c;
         C *s_prow = s_prow2[c & 1], *s_crow = s_crow2[c & 1];
