@@ -250,7 +250,10 @@ __global__ void k_fill(double *p, size_t n, double v) {
 int run_prep(stab_ctx *c, int npts, Batch &B, bool equilibrate) {
   for (int ny : B.nys) {
     double *D1 = B.ops + B.op_off[ny];
-    hipLaunchKernelGGL(k_cheb, dim3(1), dim3(256), 0, c->stream, ny, D1, D1 + ny * ny, D1 + 2 * ny * ny);
+    hipLaunchKernelGGL(k_cheb_d1, dim3(1), dim3(256), 0, c->stream, ny, D1);
+    hipLaunchKernelGGL(k_cheb_d2, dim3((ny * ny + 255) / 256), dim3(256), 0, c->stream, ny, D1, D1 + ny * ny,
+                       D1 + 2 * ny * ny);
+    hipLaunchKernelGGL(k_cheb_sums, dim3(1), dim3(256), 0, c->stream, ny, D1, D1 + ny * ny, D1 + 2 * ny * ny);
   }
   hipLaunchKernelGGL(k_nodes, dim3(npts), dim3(256), 0, c->stream, B.pts, B.ops, B.prof, B.coef, B.grid);
   hipLaunchKernelGGL(k_fill, dim3(64), dim3(256), 0, c->stream, B.norms, (size_t)4 * npts, 0.0);

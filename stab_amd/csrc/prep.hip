@@ -33,11 +33,13 @@ __global__ void k_spline(const double *__restrict__ y, const double *__restrict_
   f[nym - 1] = lam * f[nym - 2];
 }
 
-// Chebyshev operators for one ny (one block): D1 by the brute-force Lagrange
-// formula of chebyd.f90:25-57, D2 = D1 D1 (temporal.f90:136), and the last
-// row of Dt2 = D1 Dt1 with Dt1 = D1 whose wall row is zeroed
-// (temporal.f90:137-140).  Column-major, ld = ny.
-__global__ void k_cheb(int ny, double *__restrict__ D1, double *__restrict__ D2, double *__restrict__ dt2w) {
+// Chebyshev operators for one ny, in three launches: D1 by the brute-force
+// Lagrange formula of chebyd.f90:25-57 (k_cheb_d1, one block), D2 = D1 D1
+// (temporal.f90:136) and the last row of Dt2 = D1 Dt1 with Dt1 = D1 whose
+// wall row is zeroed (temporal.f90:137-140) (k_cheb_d2, one thread per entry
+// of D2, sums in ascending k), then the row sums (k_cheb_sums, one block).
+// Column-major, ld = ny.
+__global__ void k_cheb_d1(int ny, double *__restrict__ D1) {
   __shared__ double x[256], a[256];
   const int n = ny - 1;
   const double pi = acos(-1.0);
@@ -55,25 +57,33 @@ __global__ void k_cheb(int ny, double *__restrict__ D1, double *__restrict__ D2,
     for (int k = 0; k < ny; ++k)
       if (k != j) s = s + 1.0 / (x[j] - x[k]);
     D1[(size_t)j * ny + j] = s;
-    for (int k = 0; k < ny; ++k)
-      if (k != j) D1[(size_t)k * ny + j] = a[j] / (a[k] * (x[j] - x[k]));
   }
-  __syncthreads();
   for (int e = threadIdx.x; e < ny * ny; e += blockDim.x) {
-    int i = e % ny, j = e / ny;
-    double s = 0.0;
-    for (int k = 0; k < ny; ++k) s += D1[(size_t)k * ny + i] * D1[(size_t)j * ny + k];
-    D2[(size_t)j * ny + i] = s;
+    const int j = e % ny, k = e / ny;     // entry (j, k)
+    if (k != j) D1[(size_t)k * ny + j] = a[j] / (a[k] * (x[j] - x[k]));
   }
-  for (int j = threadIdx.x; j < ny; j += blockDim.x) {
-    double s = 0.0;
-    for (int k = 0; k < ny - 1; ++k) s += D1[(size_t)k * ny + (ny - 1)] * D1[(size_t)j * ny + k];
-    dt2w[j] = s;
+}
+
+// grid: ceil(ny^2 / 256) blocks of 256 threads
+__global__ void k_cheb_d2(int ny, const double *__restrict__ D1, double *__restrict__ D2, double *__restrict__ dt2w) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ny * ny) return;
+  const int i = e % ny, j = e / ny;
+  double s = 0.0;
+  for (int k = 0; k < ny; ++k) s += D1[(size_t)k * ny + i] * D1[(size_t)j * ny + k];
+  D2[(size_t)j * ny + i] = s;
+  if (i == ny - 1) {
+    double w = 0.0;
+    for (int k = 0; k < ny - 1; ++k) w += D1[(size_t)k * ny + (ny - 1)] * D1[(size_t)j * ny + k];
+    dt2w[j] = w;
   }
-  __syncthreads();
-  // row sums for the closed-form Frobenius norms of the pencil (k_norms):
-  // S11_i = sum_j D1_ij^2, S22_i = sum_j D2_ij^2, S12_i = sum_j D1_ij D2_ij,
-  // and Sdt = sum_j Dt2(wall, j)^2
+}
+
+// row sums for the closed-form Frobenius norms of the pencil (k_norms):
+// S11_i = sum_j D1_ij^2, S22_i = sum_j D2_ij^2, S12_i = sum_j D1_ij D2_ij,
+// and Sdt = sum_j Dt2(wall, j)^2, after dt2w (ny entries)
+__global__ void k_cheb_sums(int ny, const double *__restrict__ D1, const double *__restrict__ D2,
+                            double *__restrict__ dt2w) {
   double *sums = dt2w + ny;
   for (int i = threadIdx.x; i < ny; i += blockDim.x) {
     double a = 0.0, b = 0.0, c = 0.0;
