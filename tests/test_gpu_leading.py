@@ -109,6 +109,41 @@ def test_leading_invit_from_perturbed_shift(gpu_solver, sweep, krylov, mode):
     assert res["stats"]["nfallback"] == 0
 
 
+def test_leading_second_pass_on_sweep(gpu_solver):
+    """The 4096-point sweep with learned shifts (the benchmark workload): no
+    point needs the FP64 fallback, and the first six points that went through
+    the two-vector second pass (more solves than the first pass allows) return
+    the leading discrete mode the oracle's rule picks among the 8 QZ
+    eigenvalues nearest sigma = 0.33 alpha (1e-8), with the eigenvector to
+    1e-6."""
+    import bench
+    from oracle import stab_oracle as so
+    from stab_amd import _lib, learned
+    tab, al, be, re, ny = bench.make_workload(4096, 2024)
+    gpu_solver.set_profile(0, tab[:, 0], tab[:, 1:6])
+    pts = bench.make_points(al, be, re, ny, 8)
+    mlp = learned.ShiftMLP.load(gpu_solver)
+    om = learned.shifts(mlp, bench.features(al, be, re))
+    res = gpu_solver.solve_leading(pts, _lib.MODE_MIXED, _lib.LEAD_INVIT, maxit=20, fallback_shifts=om,
+                                   shifts=om, nev=4, want_vectors=True)
+    second = np.where(res["iters"] > 21)[0]
+    _say(f"sweep: nfallback {res['stats']['nfallback']}, second pass {len(second)} points")
+    assert res["stats"]["nfallback"] == 0
+    assert np.all(res["info"] == _lib.INFO_OK)
+    assert len(second) >= 6
+    for i in second[:6]:
+        A0, B0 = _oracle_pencil(tab, al[i], be[i], re[i], ny[i])
+        w, V = so.qz(A0, B0)
+        ok = np.where(np.isfinite(w) & (np.abs(w) > 1e-9))[0]
+        sig = complex(pts[i].shift.re, pts[i].shift.im)
+        near = ok[np.argsort(np.abs(w[ok] - sig))[:8]]
+        jl = so.leading_mode(1, w, V, int(ny[i]), alpha_r=float(al[i]), candidates=list(near))
+        _say(f"  pt {i} N={ny[i]} solves {res['iters'][i]} gpu {res['evals'][i]:.10f} oracle {w[jl]:.10f}")
+        assert abs(res["evals"][i] - w[jl]) <= 1e-8 * abs(w[jl])
+        x = so.normalize_evec(V[:, jl], 5 * int(ny[i]))
+        assert np.max(np.abs(res["evecs"][i] - x)) <= 1e-6
+
+
 def test_leading_invit_falls_back_from_continuum_shift(gpu_solver, sweep, krylov):
     """A shift on the continuous spectrum (omega = alpha - 1e-5 i) lands on a
     non-decaying mode; the point is redone by FP64 Krylov-Schur from the
