@@ -1060,11 +1060,11 @@ static int lead_invit(stab_ctx *c, int npts, const stab_point *pts, int mode, in
     if (f32)
       hipLaunchKernelGGL((k_invit2<float, 16, 256>), dim3(ns), dim3(256), lds2, c->stream, B.pts, B.sel,
                          (const zf *)B.mats, B.pivs, (const zf *)B.blks, B.scales, B.coef, B.ops, B.norms, 2 * maxit, tol,
-                         ev2, B.evec, rs2, ff2, B.iters, B.npadmax, B.nymax);
+                         nearest ? 1 : 0, ev2, B.evec, rs2, ff2, B.iters, B.npadmax, B.nymax);
     else
       hipLaunchKernelGGL((k_invit2<double, 16, 256>), dim3(ns), dim3(256), lds2, c->stream, B.pts, B.sel,
                          (const zd *)B.mats, B.pivs, (const zd *)B.blks, B.scales, B.coef, B.ops, B.norms, 2 * maxit, tol,
-                         ev2, B.evec, rs2, ff2, B.iters, B.npadmax, B.nymax);
+                         nearest ? 1 : 0, ev2, B.evec, rs2, ff2, B.iters, B.npadmax, B.nymax);
     HIPCHECK(c, hipGetLastError());
     HIPCHECK(c, hipEventRecord(c->ev[6], c->stream));
     std::vector<zd> e2(2 * (size_t)npts);

@@ -1103,8 +1103,11 @@ __device__ inline zd zsqrt(zd z) {
 // Starts from the vector k_invit left in column 0 of the point's block (x1)
 // and x2 = L(sigma)^-1 B x1, on the factors k_invit used.  A pair has
 // converged when its relative residual is below tol or has not halved for
-// three steps while below 1e-12; both pairs are iterated to convergence, at
-// most maxit steps (the host then takes the wanted one, see lead2_competes).
+// three steps while below 1e-12.  The wanted pair (want 0: the least stable
+// of the pairs that pass lead2_discrete; want 1: the one nearest the shift)
+// is iterated until it has converged and the other pair has converged too, or
+// has a residual below 1e-10 and can no longer outrank it (lead2_competes,
+// the test the host applies to the results); at most maxit steps.
 // grid: one work-group per listed point p = sub[blockIdx.x].  Outputs, k = 0, 1:
 // eval2[2 p + k], resid2[2 p + k], ff2[2 p + k] and the Ritz vector
 // (getevec normalisation) in column k of p's block; iters[p] += solves.
@@ -1121,12 +1124,12 @@ __host__ __device__ inline bool lead2_discrete(double alpha_r, zd ev, double ff)
 
 // Whether pair o (residual rel_o, farfield ratio ff_o), not converged, could
 // still outrank the converged pair w.  Its eigenvalue error is taken as
-// 1e4 rel_o max(|theta_o|, 1e-2) (a relative condition number of 1e4, typical
-// of the discrete modes of these pencils); its farfield ratio is trusted once
-// rel_o < 1e-8.
+// 1e5 rel_o max(|theta_o|, 1e-2) (a relative condition number of 1e5; the
+// discrete modes of these pencils stay below it); its farfield ratio is
+// trusted once rel_o < 1e-8.
 __host__ __device__ inline bool lead2_competes(int want, double alpha_r, zd sigma, zd th_w, zd th_o, double rel_o,
                                                double ff_o) {
-  const double err = 1e4 * rel_o * fmax(cabs(th_o), 1e-2);
+  const double err = 1e5 * rel_o * fmax(cabs(th_o), 1e-2);
   if (want == 1) return cabs(th_o - sigma) < cabs(th_w - sigma) + err;
   const bool maybe_discrete = !(rel_o < 1e-8) || lead2_discrete(alpha_r, th_o, ff_o);
   return maybe_discrete && th_o.y > th_w.y - err;
@@ -1137,7 +1140,7 @@ __global__ __launch_bounds__(NT) void k_invit2(const PointDev *__restrict__ pts,
                                                const cx<T> *__restrict__ mats, const int *__restrict__ pivs,
                                                const cx<T> *__restrict__ blks, const double *__restrict__ scales,
                                                const zd *__restrict__ coef, const double *__restrict__ ops,
-                                               const double *__restrict__ norms, int maxit, double tol,
+                                               const double *__restrict__ norms, int maxit, double tol, int want,
                                                zd *__restrict__ eval2, zd *__restrict__ evec,
                                                double *__restrict__ resid2, double *__restrict__ ff2,
                                                int *__restrict__ iters, int npadmax, int nymax) {
@@ -1251,12 +1254,29 @@ __global__ __launch_bounds__(NT) void k_invit2(const PointDev *__restrict__ pts,
       rel[k] = (den > 0.0) ? sqrt(nr[2 + k].x) / den : 1e300;
     }
     bool conv[2];
+    double ffk[2];
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
       if (rel[k] < 0.5 * best[k]) { best[k] = rel[k]; stall[k] = 0; } else if (rel[k] < 1e-12) { ++stall[k]; }
       conv[k] = rel[k] < tol || stall[k] >= 3;
+      ffk[k] = farfield_ratio<NT>(P, k == 0 ? x1 : x2, L.red);
     }
-    if ((conv[0] && conv[1]) || it >= maxit) break;
+    int w;
+    if (want == 1) {
+      w = (cabs(th[0] - sg) <= cabs(th[1] - sg)) ? 0 : 1;
+    } else {
+      const bool d0 = lead2_discrete(P.alpha.x, th[0], ffk[0]), d1 = lead2_discrete(P.alpha.x, th[1], ffk[1]);
+      w = (d0 && (!d1 || th[0].y >= th[1].y)) ? 0 : (d1 ? 1 : -1);
+    }
+    // early stop: the other pair is already an eigenpair to 1e-10 (so the
+    // basis holds the two eigenvalues nearest the shift, not a passing mix)
+    // and cannot outrank the converged wanted one
+    bool done = conv[0] && conv[1];
+    if (!done && w >= 0 && conv[w] && rel[w] < 1e-14 && rel[1 - w] < 1e-10) {
+      const int o = 1 - w;
+      done = !lead2_competes(want, P.alpha.x, sg, th[w], th[o], rel[o], ffk[o]);
+    }
+    if (done || it >= maxit) break;
     // corrections x_k = u_k - L(sigma)^-1 r_k: both solves in one pass over
     // the factors, in place in a1, a2 (scaled rows, zero padding)
     for (int r = t; r < P.npad; r += NT) {
