@@ -202,10 +202,14 @@ size_t stab_evec_bytes(int npts, const stab_point *pts);
  *     `mode` (STAB_MODE_MIXED: equilibrated FP32 LU) and run at most maxit
  *     residual inverse-iteration corrections x <- x - L(shift)^-1 L(lambda) x
  *     with FP64 residuals (to a relative residual of 1e-16 or the rounding
- *     floor).  A point whose residual stays above 1e-13, or that
- *     lands on a mode that is not discrete, is solved again by
- *     STAB_LEAD_KRYLOV in FP64 from fallback_shift[p] (pt.shift if NULL) and
- *     pt.nev, and reports STAB_INFO_FALLBACK.
+ *     floor).  Temporal points whose residual stays above 1e-14, that land on
+ *     a mode that is not discrete, or that needed 3/4 of maxit are iterated
+ *     again on the same factors with two vectors and 2 x 2 Rayleigh-Ritz (at
+ *     most 2 maxit steps); the wanted one of its two pairs is taken when its
+ *     residual is below 1e-14 and the other pair cannot outrank it.  A point
+ *     still undecided (or a spatial point the first pass missed) is solved
+ *     again by STAB_LEAD_KRYLOV in FP64 from fallback_shift[p] (pt.shift if
+ *     NULL) and pt.nev, and reports STAB_INFO_FALLBACK.
  *   method STAB_TRACK: as STAB_LEAD_INVIT with "nearest the shift" in place of
  *     "leading discrete": the eigenvalue selection of getab.f90:160-198 /
  *     getalpha.f90:139-145 (nearest to a guess or an extrapolation) without
