@@ -478,7 +478,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
               }
           // L21 of the chunk is loaded LH depth-4 steps at a time before the
           // MFMA chain that uses it (one memory latency per LH steps)
-          constexpr int LH = 4;
+          constexpr int LH = 8;
           for (int kb0 = 0; kb0 < kb; kb0 += 4 * LH) {
             T lr[LH][RT], li[LH][RT];
 #pragma unroll
