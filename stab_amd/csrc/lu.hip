@@ -55,18 +55,20 @@ __host__ __device__ inline long long blk_elems(int npad, int bs) {
 
 // Net permutation of the interchanges ipiv[k0 .. k0+w) (absolute rows, each
 // >= its step): touched rows srow[e] receive the input row sorig[e];
-// returns the number of touched rows (<= 2 w).  One thread.
-__device__ inline int net_perm(const int *ipiv, int k0, int w, int *srow, int *sorig) {
+// returns the number of touched rows (<= 2 w).  One thread; slot[] (one
+// entry per row, -1 when the row is not in the list) finds a row's entry in
+// O(1) and is -1 again on return.
+__device__ inline int net_perm(const int *ipiv, int k0, int w, int *srow, int *sorig, int *slot) {
   int ns = 0;
   for (int c = 0; c < w; ++c) {
     const int a = k0 + c, b = ipiv[c];
     if (a == b) continue;
-    int ia = -1, ib = -1;
-    for (int e = 0; e < ns; ++e) { if (srow[e] == a) ia = e; if (srow[e] == b) ib = e; }
-    if (ia < 0) { ia = ns; srow[ns] = a; sorig[ns] = a; ++ns; }
-    if (ib < 0) { ib = ns; srow[ns] = b; sorig[ns] = b; ++ns; }
+    int ia = slot[a], ib = slot[b];
+    if (ia < 0) { ia = ns; srow[ns] = a; sorig[ns] = a; slot[a] = ns; ++ns; }
+    if (ib < 0) { ib = ns; srow[ns] = b; sorig[ns] = b; slot[b] = ns; ++ns; }
     const int tmp = sorig[ia]; sorig[ia] = sorig[ib]; sorig[ib] = tmp;
   }
+  for (int e = 0; e < ns; ++e) slot[srow[e]] = -1;
   return ns;
 }
 
@@ -95,6 +97,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
   __shared__ C s_linv[NPB][NB * NB];      // inv(L11) of each panel of the block
   __shared__ int s_piv[KB];
   __shared__ int s_srow[2 * KB], s_sorig[2 * KB];
+  __shared__ int s_slot[1280];            // net_perm's row map (npad <= 1280)
   __shared__ int s_topsrc[KB], s_outrow[KB], s_outsrc[KB];
   __shared__ int s_nout, s_ns;
   __shared__ C s_stage[2 * NB * SW];
@@ -103,6 +106,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
 
   C Pn[R][NB];
   int myinfo = 0;
+  for (int r = t; r < 1280; r += NT) s_slot[r] = -1;
+  __syncthreads();
 
   for (int K0 = 0; K0 < n; K0 += KB) {
     const int kb = min(KB, n - K0);       // 64, or 32 for the last block (n is a multiple of 32)
@@ -236,7 +241,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
       // ---- net permutation of the panel's interchanges: rows of the panel
       // top and the rows below it that change
       if (t == 0) {
-        const int ns = net_perm(s_piv + (k0 - K0), k0, NB, s_srow, s_sorig);
+        const int ns = net_perm(s_piv + (k0 - K0), k0, NB, s_srow, s_sorig, s_slot);
         for (int c = 0; c < NB; ++c) s_topsrc[c] = k0 + c;
         int no = 0;
         for (int e = 0; e < ns; ++e) {
@@ -328,7 +333,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
     // input rows s_topsrc[], rows below take s_outsrc[] (always input rows of
     // the block top); s_srow/s_sorig list every touched row
     if (t == 0) {
-      const int ns = net_perm(s_piv, K0, kb, s_srow, s_sorig);
+      const int ns = net_perm(s_piv, K0, kb, s_srow, s_sorig, s_slot);
       for (int c = 0; c < kb; ++c) s_topsrc[c] = K0 + c;
       int no = 0;
       for (int e = 0; e < ns; ++e) {
