@@ -532,16 +532,22 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
       }
     }
   }  // blocks
-  // ---- final row permutation: row r of L U is row perm[r] of the input
+  // ---- final row permutation: row r of L U is row perm[r] of the input,
+  // composed in LDS (s_slot, idle now) by one thread, then stored
   int *perm = ipiv + n;
-  for (int r = t; r < n; r += NT) perm[r] = r;
+  static_assert(sizeof(s_u) >= 1280 * sizeof(int), "the pivots fit in s_u");
+  int *s_ip = reinterpret_cast<int *>(s_u);   // the pivots, read by one thread
+  __syncthreads();
+  for (int r = t; r < n; r += NT) { s_slot[r] = r; s_ip[r] = ipiv[r]; }
   __syncthreads();
   if (t == 0) {
     for (int c = 0; c < n; ++c) {
-      const int p = ipiv[c];
-      if (p != c) { const int a = perm[c]; perm[c] = perm[p]; perm[p] = a; }
+      const int p = s_ip[c];
+      if (p != c) { const int a = s_slot[c]; s_slot[c] = s_slot[p]; s_slot[p] = a; }
     }
   }
+  __syncthreads();
+  for (int r = t; r < n; r += NT) perm[r] = s_slot[r];
   if (t == 0) info[blockIdx.x] = myinfo;
 }
 
