@@ -201,7 +201,7 @@ size_t stab_evec_bytes(int npts, const stab_point *pts);
  *   method STAB_LEAD_INVIT: factor at pt.shift (a predicted eigenvalue) in
  *     `mode` (STAB_MODE_MIXED: equilibrated FP32 LU) and run at most maxit
  *     residual inverse-iteration corrections x <- x - L(shift)^-1 L(lambda) x
- *     with FP64 residuals (to a relative residual of 1e-16 or the rounding
+ *     with FP64 residuals (to a relative residual of 1e-15 or the rounding
  *     floor).  Temporal points whose residual stays above 1e-14, that land on
  *     a mode that is not discrete, or that needed 3/4 of maxit are iterated
  *     again on the same factors with two vectors and 2 x 2 Rayleigh-Ritz (at
