@@ -561,7 +561,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
 // q of column c sums the terms k = q (mod 4) of each substitution step and
 // keeps the entries x[k][c], k = q (mod 4), in registers; the step's sum is
 // reduced over the four lanes by shuffles, so the 64-step recurrence needs
-// no barrier.  Only the block itself lives in LDS.
+// no barrier.  Only the block itself lives in LDS, as stored (the arithmetic
+// converts to FP64).
 template <typename T, int BS>
 __global__ __launch_bounds__(4 * BS) void k_blkinv(const PointDev *__restrict__ pts, const cx<T> *__restrict__ mats,
                                                    cx<T> *__restrict__ blks) {
@@ -573,10 +574,11 @@ __global__ __launch_bounds__(4 * BS) void k_blkinv(const PointDev *__restrict__ 
   const int nblk = (n + BS - 1) / BS;
   const cx<T> *A = mats + P.mat_off;
   cx<T> *Ui = blks + P.blk_off + (size_t)(nblk + blockIdx.x) * BS * BS;
-  __shared__ zd s_a[BS * BS];     // row-major copy of the block
+  constexpr int LD = BS + 1;       // padded rows: the column-wise staging writes hit distinct banks
+  __shared__ cx<T> s_a[BS * LD];  // row-major copy of the block, as stored (FP32: 33 KB, so more groups per CU)
   for (int e = t; e < BS * BS; e += 4 * BS) {
-    const int i = e / BS, j = e % BS;
-    s_a[e] = (i < bs && j < bs) ? cvt<double>(A[(size_t)(k0 + j) * n + k0 + i]) : czero<double>();
+    const int i = e % BS, j = e / BS;   // consecutive lanes read consecutive rows of column j
+    s_a[i * LD + j] = (i < bs && j < bs) ? A[(size_t)(k0 + j) * n + k0 + i] : czero<T>();
   }
   __syncthreads();
   const int c = t >> 2, q = t & 3;
@@ -593,12 +595,12 @@ __global__ __launch_bounds__(4 * BS) void k_blkinv(const PointDev *__restrict__ 
 #pragma unroll
       for (int m = mi; m < BS / 4; ++m) {
         const int k = 4 * m + q;
-        if (k > i && k <= c) s = cfma(s, s_a[i * BS + k], x[m]);
+        if (k > i && k <= c) s = cfma(s, cvt<double>(s_a[i * LD + k]), x[m]);
       }
       s.x += __shfl_xor(s.x, 1); s.y += __shfl_xor(s.y, 1);
       s.x += __shfl_xor(s.x, 2); s.y += __shfl_xor(s.y, 2);
       const zd rhs = ((i == c) ? mk(1.0, 0.0) : czero<double>()) - s;
-      const zd v = (i > c || c >= bs) ? czero<double>() : cdiv(rhs, s_a[i * BS + i]);
+      const zd v = (i > c || c >= bs) ? czero<double>() : cdiv(rhs, cvt<double>(s_a[i * LD + i]));
       if (r == q) x[mi] = v;
     }
   }
