@@ -236,11 +236,12 @@ def evaluate(S: sv.Solver, mlp: ShiftMLP, pts, feats, ref, maxit=20):
 def main(argv=None):
     ap = argparse.ArgumentParser(description="generate labels with the plain path, train the shift MLP on the GPU, "
                                              "evaluate it on held-out points, save the weights")
-    ap.add_argument("--train-points", type=int, default=200000)
+    ap.add_argument("--train-points", type=int, default=400000)
     ap.add_argument("--eval-points", type=int, default=8192)
     ap.add_argument("--ny-train", type=int, default=96)
     ap.add_argument("--ny-eval", type=int, default=128)
-    ap.add_argument("--steps", type=int, default=60000)
+    ap.add_argument("--steps", type=int, default=120000)
+    ap.add_argument("--width", type=int, default=256)
     ap.add_argument("--seed", type=int, default=11)
     ap.add_argument("--workers", type=int, default=8)
     ap.add_argument("--out", default=WEIGHTS)
@@ -261,7 +262,7 @@ def main(argv=None):
     s1 = int(a.steps * 0.5)
     sched = ((s1, 1e-3), (int(a.steps * 0.25), 3e-4), (a.steps - s1 - int(a.steps * 0.25), 1e-4))
     t0 = time.perf_counter()
-    mlp, hist = fit(S, feats[ok], targets(feats[ok], om[ok]), seed=a.seed, schedule=sched)
+    mlp, hist = fit(S, feats[ok], targets(feats[ok], om[ok]), width=a.width, seed=a.seed, schedule=sched)
     tt = time.perf_counter() - t0
     print(f"# trained {a.steps} steps in {tt:.1f} s, loss {hist[:10].mean():.3e} -> {hist[-100:].mean():.3e}",
           flush=True)
@@ -270,7 +271,7 @@ def main(argv=None):
     ev = evaluate(S, mlp, ept, ef, ref)
     ev.pop("result")
     print(f"# held-out: {json.dumps(ev)}", flush=True)
-    meta = dict(train_points=a.train_points, labelled=int(ok.sum()), ny_train=a.ny_train, steps=a.steps,
+    meta = dict(train_points=a.train_points, labelled=int(ok.sum()), ny_train=a.ny_train, steps=a.steps, width=a.width,
                 seed=a.seed, family=dict(machs=list(fam.machs), tws=list(fam.tws)), loss_final=float(hist[-100:].mean()),
                 heldout=dict((k, v) for k, v in ev.items() if k != "stats"))
     mlp.save(a.out, meta)
