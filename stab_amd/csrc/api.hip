@@ -120,6 +120,7 @@ inline int round_up(int a, int b) { return (a + b - 1) / b * b; }
 // Layout of one batch on the device.
 struct Batch {
   std::vector<PointDev> h;
+  std::vector<int> horder;
   PointDev *pts = nullptr;
   double *ops = nullptr, *prof = nullptr, *grid = nullptr, *scales = nullptr, *norms = nullptr;
   zd *coef = nullptr, *V = nullptr, *small = nullptr, *evec = nullptr, *eval = nullptr;
@@ -129,6 +130,7 @@ struct Batch {
   long long *doff = nullptr;
   zd *lead_vec = nullptr;
   int *pivs = nullptr, *info = nullptr;
+  int *order = nullptr;   // points by decreasing order npad (work-group -> point in k_lu, k_invit)
   KSState *state = nullptr;
   int npadmax = 0, nymax = 0, nvecmax = 0;
   size_t nscales = 0;
@@ -227,6 +229,7 @@ int build_batch(stab_ctx *c, int npts, const stab_point *pts, size_t elem, int m
   GET("sel", sizeof(int) * npts, B.sel);
   GET("flag", sizeof(int) * npts, B.flag);
   GET("doff", sizeof(long long) * (npts + 1), B.doff);
+  GET("order", sizeof(int) * npts, B.order);
   {
     long long tot = 0;
     for (int i = 0; i < npts; ++i) tot += B.h[i].nvec;
@@ -234,6 +237,12 @@ int build_batch(stab_ctx *c, int npts, const stab_point *pts, size_t elem, int m
   }
 #undef GET
   HIPCHECK(c, hipMemcpyAsync(B.pts, B.h.data(), sizeof(PointDev) * npts, hipMemcpyHostToDevice, c->stream));
+  // Largest matrices first: work-groups start roughly in index order, so the
+  // batch then ends on its cheapest points instead of on a few large ones.
+  B.horder.resize(npts);
+  for (int i = 0; i < npts; ++i) B.horder[i] = i;
+  std::stable_sort(B.horder.begin(), B.horder.end(), [&](int a, int b) { return B.h[a].npad > B.h[b].npad; });
+  HIPCHECK(c, hipMemcpyAsync(B.order, B.horder.data(), sizeof(int) * npts, hipMemcpyHostToDevice, c->stream));
   for (auto &kv : prof_off) {
     const Profile &pr = c->profiles[kv.first];
     HIPCHECK(c, hipMemcpyAsync(B.prof + kv.second, pr.data.data(), sizeof(double) * pr.data.size(),
@@ -270,9 +279,9 @@ int launch_lu_nt(stab_ctx *c, int npts, Batch &B) {
   const int R = (B.npadmax + NT - 1) / NT;
   cx<T> *mats = (cx<T> *)B.mats, *blks = (cx<T> *)B.blks;
   switch (R) {
-    case 1: hipLaunchKernelGGL((k_lu<T, NB, NT, 1, 16>), dim3(npts), dim3(NT), 0, c->stream, B.pts, mats, B.pivs, blks, B.info, nullptr); break;
-    case 2: hipLaunchKernelGGL((k_lu<T, NB, NT, 2, 16>), dim3(npts), dim3(NT), 0, c->stream, B.pts, mats, B.pivs, blks, B.info, nullptr); break;
-    case 3: hipLaunchKernelGGL((k_lu<T, NB, NT, 3, 16>), dim3(npts), dim3(NT), 0, c->stream, B.pts, mats, B.pivs, blks, B.info, nullptr); break;
+    case 1: hipLaunchKernelGGL((k_lu<T, NB, NT, 1, 16>), dim3(npts), dim3(NT), 0, c->stream, B.pts, mats, B.pivs, blks, B.info, nullptr, B.order); break;
+    case 2: hipLaunchKernelGGL((k_lu<T, NB, NT, 2, 16>), dim3(npts), dim3(NT), 0, c->stream, B.pts, mats, B.pivs, blks, B.info, nullptr, B.order); break;
+    case 3: hipLaunchKernelGGL((k_lu<T, NB, NT, 3, 16>), dim3(npts), dim3(NT), 0, c->stream, B.pts, mats, B.pivs, blks, B.info, nullptr, B.order); break;
     default: return fail(c, STAB_E_ARG, "matrix order %d too large for the LU kernel", B.npadmax);
   }
   HIPCHECK(c, hipGetLastError());
@@ -1010,11 +1019,11 @@ static int lead_invit(stab_ctx *c, int npts, const stab_point *pts, int mode, in
   if (f32)
     hipLaunchKernelGGL((k_invit<float, 16, 256>), dim3(npts), dim3(256), lds, c->stream, B.pts, (const zf *)B.mats,
                        B.pivs, (const zf *)B.blks, B.scales, B.coef, B.ops, B.norms, maxit, tol, B.eval, B.evec,
-                       B.resid, B.ff, B.iters, B.flag, B.npadmax, B.nymax);
+                       B.resid, B.ff, B.iters, B.flag, B.npadmax, B.nymax, B.order);
   else
     hipLaunchKernelGGL((k_invit<double, 16, 256>), dim3(npts), dim3(256), lds, c->stream, B.pts, (const zd *)B.mats,
                        B.pivs, (const zd *)B.blks, B.scales, B.coef, B.ops, B.norms, maxit, tol, B.eval, B.evec,
-                       B.resid, B.ff, B.iters, B.flag, B.npadmax, B.nymax);
+                       B.resid, B.ff, B.iters, B.flag, B.npadmax, B.nymax, B.order);
   HIPCHECK(c, hipGetLastError());
   HIPCHECK(c, hipEventRecord(c->ev[4], c->stream));
   std::vector<zd> ev(npts);
@@ -1050,6 +1059,7 @@ static int lead_invit(stab_ctx *c, int npts, const stab_point *pts, int mode, in
   std::vector<int> sel(npts, 0), sub;
   for (int i = 0; i < npts; ++i)
     if (!inf[i] && pts[i].itype == 1 && (!ok[i] || 4 * it[i] >= 3 * (maxit + 1))) sub.push_back(i);
+  std::stable_sort(sub.begin(), sub.end(), [&](int a, int b) { return B.h[a].npad > B.h[b].npad; });
   const size_t lds2 = lds + sizeof(zd) * (6 * (size_t)B.npadmax + 2 * 256);
   float ms2 = 0.f;
   if (!sub.empty() && lds2 <= 160 * 1024) {

@@ -939,9 +939,10 @@ __global__ __launch_bounds__(NT) void k_invit(const PointDev *__restrict__ pts, 
                                               int maxit, double tol, zd *__restrict__ eval, zd *__restrict__ evec,
                                               double *__restrict__ resid, double *__restrict__ ffo,
                                               int *__restrict__ iters, int *__restrict__ flag, int npadmax,
-                                              int nymax) {
+                                              int nymax, const int *__restrict__ order) {
   extern __shared__ zd lds[];
-  const PointDev &P = pts[blockIdx.x];
+  const int pi = order ? order[blockIdx.x] : blockIdx.x;   // largest points first
+  const PointDev &P = pts[pi];
   OpLds L(lds, P, npadmax, nymax, 1, npadmax);
   const int t = threadIdx.x, n = P.n;
   const cx<T> *A = mats + P.mat_off;
@@ -949,15 +950,15 @@ __global__ __launch_bounds__(NT) void k_invit(const PointDev *__restrict__ pts, 
   const cx<T> *Lb = blks + P.blk_off, *Ub = nullptr;
   const double *dr = scales + P.scale_off, *dc = dr + P.npad;
   const bool temporal = (P.itype == 1);
-  const double nA = sqrt(norms[blockIdx.x * 4 + 0]), nB = sqrt(norms[blockIdx.x * 4 + 1]),
-               nC = sqrt(norms[blockIdx.x * 4 + 2]);
+  const double nA = sqrt(norms[pi * 4 + 0]), nB = sqrt(norms[pi * 4 + 1]),
+               nC = sqrt(norms[pi * 4 + 2]);
   const zd sg = P.shift;
   zd *x = L.z;
   __shared__ double s_mx[NT];
   __shared__ int s_ix[NT];
 
   // ---- start: x = L(sigma)^-1 L'(sigma) r
-  for (int r = t; r < n; r += NT) x[r] = mk(hash01(blockIdx.x, 2 * r), hash01(blockIdx.x, 2 * r + 1));
+  for (int r = t; r < n; r += NT) x[r] = mk(hash01(pi, 2 * r), hash01(pi, 2 * r + 1));
   __syncthreads();
   if (temporal) {
     pencil_apply<NT>(P, coef, ops, FORM_BM, sg, x, L.rhs, L.X1, L.X2, L.red);
@@ -1072,11 +1073,11 @@ __global__ __launch_bounds__(NT) void k_invit(const PointDev *__restrict__ pts, 
     if (!temporal) out[n + r] = lam * x[r];
   }
   if (t == 0) {
-    eval[blockIdx.x] = lam;
-    resid[blockIdx.x] = rel;
-    ffo[blockIdx.x] = ff;
-    iters[blockIdx.x] = nsolve;
-    flag[blockIdx.x] = (rel < 1e-14) ? 0 : 2;
+    eval[pi] = lam;
+    resid[pi] = rel;
+    ffo[pi] = ff;
+    iters[pi] = nsolve;
+    flag[pi] = (rel < 1e-14) ? 0 : 2;
   }
 }
 

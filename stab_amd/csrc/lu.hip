@@ -76,10 +76,14 @@ __device__ inline int net_perm(const int *ipiv, int k0, int w, int *srow, int *s
 template <typename T, int NB, int NT, int R, int SW>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_lu(const PointDev *__restrict__ pts, cx<T> *__restrict__ mats,
                                            int *__restrict__ pivs, cx<T> *__restrict__ blks,
-                                           int *__restrict__ info, const int *__restrict__ active) {
+                                           int *__restrict__ info, const int *__restrict__ active,
+                                           const int *__restrict__ order) {
   typedef cx<T> C;
-  if (active && !active[blockIdx.x]) return;
-  const PointDev &P = pts[blockIdx.x];
+  // work-group -> point: `order` lists the points largest first, so the
+  // batch's last work-groups are its cheapest
+  const int pi = order ? order[blockIdx.x] : blockIdx.x;
+  if (active && !active[pi]) return;
+  const PointDev &P = pts[pi];
   const int n = P.npad, lda = P.npad;
   C *A = mats + P.mat_off;
   int *ipiv = pivs + P.piv_off;
@@ -548,7 +552,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
   }
   __syncthreads();
   for (int r = t; r < n; r += NT) perm[r] = s_slot[r];
-  if (t == 0) info[blockIdx.x] = myinfo;
+  if (t == 0) info[pi] = myinfo;
 }
 
 // Inverses of the diagonal blocks of U (upper) of order BS = 64 (the last
