@@ -118,7 +118,23 @@ __device__ void pencil_apply(const PointDev &P, const zd *__restrict__ coef, con
   const double *D1 = ops + P.op_off, *D2 = D1 + (size_t)ny * ny, *dt2w = D2 + (size_t)ny * ny;
   const bool need_d2 = (form == FORM_A0 || form == FORM_AM || form == FORM_C0 || form == FORM_SHIFTED);
   const bool need_d1 = (form != FORM_B0 && form != FORM_BM && form != FORM_C2);
-  if (need_d1) {
+  if (need_d1 && need_d2 && 2 * ny <= NT) {
+    // one half of the work-group forms X1 = D1 x, the other X2 = D2 x, one
+    // node row per thread
+    const int g = (t >= NT / 2) ? 1 : 0, i = t - g * (NT / 2);
+    if (i < ny) {
+      const double *D = g ? D2 : D1;
+      zd a[5];
+      for (int k = 0; k < 5; ++k) a[k] = czero<double>();
+#pragma unroll 8
+      for (int j = 0; j < ny; ++j) {
+        const double d = D[(size_t)j * ny + i];
+        for (int k = 0; k < 5; ++k) a[k] = a[k] + x[j * 5 + k] * d;
+      }
+      zd *X = g ? X2 : X1;
+      for (int k = 0; k < 5; ++k) X[i * 5 + k] = a[k];
+    }
+  } else if (need_d1) {
     for (int i = t; i < ny; i += NT) {
       zd a[5], b[5];
       for (int k = 0; k < 5; ++k) { a[k] = czero<double>(); b[k] = czero<double>(); }
@@ -134,6 +150,8 @@ __device__ void pencil_apply(const PointDev &P, const zd *__restrict__ coef, con
       }
       for (int k = 0; k < 5; ++k) { X1[i * 5 + k] = a[k]; X2[i * 5 + k] = b[k]; }
     }
+  }
+  if (need_d1) {
     if (P.wallt == 2 && t < 64) {
       // Dt2(wall,:) . x_T  (temporal.f90:137-140): one wave reduces
       zd s = czero<double>();
