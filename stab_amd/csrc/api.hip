@@ -191,7 +191,7 @@ int build_batch(stab_ctx *c, int npts, const stab_point *pts, size_t elem, int m
     d.mat_off = moff; moff += (long long)d.npad * d.npad;
     d.vec_off = voff; voff += (long long)d.nvec * (m + 1);
     d.small_off = (long long)i * SM_SIZE;
-    d.piv_off = pvoff; pvoff += 2LL * d.npad;   // ipiv then perm
+    d.piv_off = pvoff; pvoff += piv_elems(d.npad);   // ipiv, perm, block interchange records
     d.blk_off = boff; boff += blk_elems(d.npad, SOLVE_BS);
     d.scale_off = soff; soff += 2LL * d.npad;
     d.out_off = eoff; eoff += (long long)d.nvec * STAB_MAX_NEV;
@@ -275,24 +275,28 @@ int run_prep(stab_ctx *c, int npts, Batch &B, bool equilibrate) {
 }
 
 template <typename T, int NB, int NT>
-int launch_lu_nt(stab_ctx *c, int npts, Batch &B) {
+int launch_lu_nt(stab_ctx *c, int npts, Batch &B, int left) {
   const int R = (B.npadmax + NT - 1) / NT;
   cx<T> *mats = (cx<T> *)B.mats, *blks = (cx<T> *)B.blks;
   switch (R) {
-    case 1: hipLaunchKernelGGL((k_lu<T, NB, NT, 1, 16>), dim3(npts), dim3(NT), 0, c->stream, B.pts, mats, B.pivs, blks, B.info, nullptr, B.order); break;
-    case 2: hipLaunchKernelGGL((k_lu<T, NB, NT, 2, 16>), dim3(npts), dim3(NT), 0, c->stream, B.pts, mats, B.pivs, blks, B.info, nullptr, B.order); break;
-    case 3: hipLaunchKernelGGL((k_lu<T, NB, NT, 3, 16>), dim3(npts), dim3(NT), 0, c->stream, B.pts, mats, B.pivs, blks, B.info, nullptr, B.order); break;
+    case 1: hipLaunchKernelGGL((k_lu<T, NB, NT, 1, 16>), dim3(npts), dim3(NT), 0, c->stream, B.pts, mats, B.pivs, blks, B.info, nullptr, B.order, left); break;
+    case 2: hipLaunchKernelGGL((k_lu<T, NB, NT, 2, 16>), dim3(npts), dim3(NT), 0, c->stream, B.pts, mats, B.pivs, blks, B.info, nullptr, B.order, left); break;
+    case 3: hipLaunchKernelGGL((k_lu<T, NB, NT, 3, 16>), dim3(npts), dim3(NT), 0, c->stream, B.pts, mats, B.pivs, blks, B.info, nullptr, B.order, left); break;
     default: return fail(c, STAB_E_ARG, "matrix order %d too large for the LU kernel", B.npadmax);
   }
   HIPCHECK(c, hipGetLastError());
   return STAB_OK;
 }
 
+// lapack_form: L in final row order (stab_lu_factors); otherwise the
+// interchanges of each block are left out of the columns before it and the
+// solves apply them block by block.
 template <typename T, int NB>
-int launch_lu(stab_ctx *c, int npts, Batch &B) {
+int launch_lu(stab_ctx *c, int npts, Batch &B, bool lapack_form = false) {
   // 4 waves (two work-groups per CU) up to order 768, 8 waves up to 1280;
   // at most three panel rows per thread
-  int rc = (B.npadmax <= 768) ? launch_lu_nt<T, NB, 256>(c, npts, B) : launch_lu_nt<T, NB, 512>(c, npts, B);
+  const int left = lapack_form ? 1 : 0;
+  int rc = (B.npadmax <= 768) ? launch_lu_nt<T, NB, 256>(c, npts, B, left) : launch_lu_nt<T, NB, 512>(c, npts, B, left);
   if (rc != STAB_OK) return rc;
   HIPCHECK(c, hipEventRecord(c->ev[7], c->stream));   // end of k_lu (its start is ev[2] in the solve paths)
   hipLaunchKernelGGL((k_blkinv<T, SOLVE_BS>), dim3((B.npadmax + SOLVE_BS - 1) / SOLVE_BS, npts), dim3(4 * SOLVE_BS), 0,
@@ -681,7 +685,7 @@ int stab_lu_factors(stab_ctx *c, const stab_point *pt, int mode, stab_z *lu, int
   else
     hipLaunchKernelGGL((k_assemble<double, 8>), dim3(cb, 1), dim3(256), 0, c->stream, B.pts, B.coef, B.ops,
                        B.scales, (zd *)B.mats, (int)FORM_SHIFTED, 0);
-  if ((rc = f32 ? launch_lu<float, 16>(c, 1, B) : launch_lu<double, 8>(c, 1, B)) != STAB_OK) return rc;
+  if ((rc = f32 ? launch_lu<float, 16>(c, 1, B, true) : launch_lu<double, 8>(c, 1, B, true)) != STAB_OK) return rc;
   const int np = B.h[0].npad;
   std::vector<int> pv(2 * (size_t)np);
   std::vector<double> sc(2 * (size_t)np);

@@ -91,6 +91,20 @@ struct PointDev {
   long long out_off;             // offset (zd) of the output eigenvectors
 };
 
+// Pivot storage of one point (ints at piv_off): ipiv[npad] (absolute rows,
+// as ZGETRF), perm[npad] (row r of L U is row perm[r] of the input), then a
+// mode word and one record per 64-column block of the LU:
+//   mode 0: every interchange has also been applied to the columns left of
+//           its block (L in final row order, the LAPACK form);
+//   mode 1: it has not: the L columns of block b keep the row order reached
+//           after block b, and a forward solve applies each block's net
+//           interchanges to the right-hand side before that block
+//           (LINPACK's order of work).  Record b: ns, then ns touched rows
+//           srow[] at +1 and their source rows sorig[] at +1+128:
+//           y[srow[e]] <- y[sorig[e]].
+#define LU_PREC 257
+__host__ __device__ inline long long piv_elems(int npad) { return 2LL * npad + 1 + (long long)LU_PREC * ((npad + 63) / 64); }
+
 // Matrix forms produced by the assembly kernel
 enum {
   FORM_A0 = 0, FORM_B0 = 1, FORM_C0 = 2, FORM_C1 = 3, FORM_C2 = 4, FORM_SHIFTED = 5,

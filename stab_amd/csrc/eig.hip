@@ -161,21 +161,41 @@ template <typename T, int NT, int NR>
 __device__ __forceinline__ void lu_solve_n(const PointDev &P, const cx<T> *__restrict__ A, const int *__restrict__ ipiv,
                                            const cx<T> *__restrict__ blk, zd *const (&y)[NR], zd *tmp, zd *work) {
   static_assert(NT == 4 * SOLVE_BS, "four lanes per row of a 64-row block");
+  static_assert(LU_PREC == 1 + 4 * SOLVE_BS, "one interchange record per 64-row block");
   constexpr int BS = SOLVE_BS;
   const int n = P.npad, lda = P.npad, t = threadIdx.x;
   const int *perm = ipiv + n;
   const int nblk = (n + BS - 1) / BS;
   const cx<T> *Li = blk, *Ui = blk + (size_t)nblk * BS * BS;   // layout of k_blkinv
-  // y <- P y (one gather through `work` per right-hand side)
+  const int *rec0 = ipiv + 2 * n;                                // pivot mode, block records
+  const bool progressive = rec0[0] != 0;
+  if (!progressive) {
+    // y <- P y (one gather through `work` per right-hand side)
 #pragma unroll
-  for (int r = 0; r < NR; ++r) {
-    for (int i = t; i < n; i += NT) work[i] = y[r][perm[i]];
-    __syncthreads();
-    for (int i = t; i < n; i += NT) y[r][i] = work[i];
-    __syncthreads();
+    for (int r = 0; r < NR; ++r) {
+      for (int i = t; i < n; i += NT) work[i] = y[r][perm[i]];
+      __syncthreads();
+      for (int i = t; i < n; i += NT) y[r][i] = work[i];
+      __syncthreads();
+    }
   }
   for (int b = 0; b < nblk; ++b) {
     const int k0 = b * BS, bs = min(BS, n - k0);
+    if (progressive) {
+      // the block's net interchanges (rows >= k0, at most 2 BS of them and
+      // never more than n - k0, so they fit in work)
+      const int *rec = rec0 + 1 + (size_t)b * LU_PREC;
+      const int ns = rec[0];
+      if (ns > 0) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+          for (int e = t; e < ns; e += NT) work[e] = y[r][rec[1 + 2 * BS + e]];
+          __syncthreads();
+          for (int e = t; e < ns; e += NT) y[r][rec[1 + e]] = work[e];
+          __syncthreads();
+        }
+      }
+    }
     solve_diag<T, NT, NR>(Li + (size_t)b * BS * BS, k0, bs, y, tmp);
     if (k0 + bs < n) {
       solve_update<T, NT, NR>(A, lda, k0, bs, k0 + bs, n, y, tmp);

@@ -77,7 +77,7 @@ template <typename T, int NB, int NT, int R, int SW>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_lu(const PointDev *__restrict__ pts, cx<T> *__restrict__ mats,
                                            int *__restrict__ pivs, cx<T> *__restrict__ blks,
                                            int *__restrict__ info, const int *__restrict__ active,
-                                           const int *__restrict__ order) {
+                                           const int *__restrict__ order, int left) {
   typedef cx<T> C;
   // work-group -> point: `order` lists the points largest first, so the
   // batch's last work-groups are its cheapest
@@ -91,6 +91,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
   constexpr int NW = NT / 64;
   constexpr int KB = 64, GW = lu_gw<T>(), UP = GW + 1, NPB = KB / NB;
   static_assert(KB % NB == 0, "panel width must divide the block width");
+  static_assert(LU_PREC == 1 + 4 * KB, "interchange record: count, 2 KB rows, 2 KB sources");
 
   // double-buffered by column parity: a column's slots are rewritten two
   // columns later, after a barrier every wave passes only once it has read them
@@ -350,9 +351,16 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
     }
     __syncthreads();
     const int bns = s_ns, bnout = s_nout;
-    // ---- left columns [0, K0): move the touched rows, 16 columns at a time,
-    // through LDS (s_u is idle here; bns * 16 <= 2 KB * 16 entries fit)
-    if (bns > 0 && K0 > 0) {
+    if (!left) {
+      // ---- the block's net interchanges for the forward solves (pivot mode 1)
+      int *rec = ipiv + 2 * n + 1 + (size_t)(K0 / KB) * LU_PREC;
+      for (int e = t; e < bns; e += NT) { rec[1 + e] = s_srow[e]; rec[1 + 2 * KB + e] = s_sorig[e]; }
+      if (t == 0) rec[0] = bns;
+    }
+    // ---- left columns [0, K0) (pivot mode 0 only): move the touched rows,
+    // 16 columns at a time, through LDS (s_u is idle here; bns * 16 <= 2 KB * 16
+    // entries fit)
+    if (left && bns > 0 && K0 > 0) {
       for (int c0 = 0; c0 < K0; c0 += 16) {
         for (int e = t; e < bns * 16; e += NT) s_u[e] = A[(size_t)(c0 + e / bns) * lda + s_sorig[e % bns]];
         __syncthreads();
@@ -552,7 +560,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
   }
   __syncthreads();
   for (int r = t; r < n; r += NT) perm[r] = s_slot[r];
-  if (t == 0) info[pi] = myinfo;
+  if (t == 0) { info[pi] = myinfo; ipiv[2 * n] = left ? 0 : 1; }
 }
 
 // Inverses of the diagonal blocks of U (upper) of order BS = 64 (the last
