@@ -127,6 +127,31 @@ def test_shifted_solve(gpu_solver, ts_profile, mode, ir, tol):
     assert err < tol
 
 
+@pytest.mark.parametrize("mode,ir,tol", [(0, 0, 1e-13), (1, -12, 1e-13)])
+@pytest.mark.parametrize("ny", [8, 70, 160])
+def test_shifted_solve_orders(gpu_solver, ts_profile, ny, mode, ir, tol):
+    """Solves whose LU leaves each block's interchanges out of the columns
+    before it (the forward solve applies them block by block) at orders 64
+    (one block), 352 (last block 32 wide) and 800 (the 512-thread LU): backward
+    error ||b - M x|| / (||M|| ||x||), which a misplaced interchange would
+    raise to O(1)."""
+    from stab_amd import solver as sv
+    from stab_amd import _lib
+    _say(f"shifted solve ny {ny} mode {mode} ir {ir}")
+    text = open(golden("thesis_TStest", "temporal.inp")).read().replace("96 1.0 0.0", f"{ny} 1.0 0.0")
+    inp = sv.read_input(text)
+    gpu_solver.set_profile(0, *ts_profile)
+    pt = sv.make_point(inp, 0.2 + 0.05j, 4)
+    M = gpu_solver.pencil(pt, _lib.MAT_SHIFTED)
+    rng = np.random.default_rng(ny)
+    b = rng.standard_normal(M.shape[0]) + 1j * rng.standard_normal(M.shape[0])
+    x, info = gpu_solver.shifted_solve(pt, b, mode, ir)
+    berr = np.linalg.norm(b - M @ x) / (np.linalg.norm(M, 2) * np.linalg.norm(x))
+    _say(f"  info {info} backward err {berr:.2e}")
+    assert info == 0
+    assert berr < tol
+
+
 @pytest.mark.parametrize("mode,ir,tol", [(0, 0, 1e-15), (1, 0, 1e-6), (1, 2, 1e-11), (1, -12, 1e-15)])
 def test_shifted_solve_near_eigenvalue(gpu_solver, ts_profile, mode, ir, tol):
     """Shift within 1e-4 of an eigenvalue: backward error ||b - M x|| / (||M|| ||x||)."""
