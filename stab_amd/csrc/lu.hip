@@ -19,10 +19,12 @@
 //     runs over matrix columns) so that each accumulator register of 16 lanes
 //     covers 16 consecutive matrix rows and loads/stores coalesce.
 // The trailing matrix is read and written once per 64 columns.  P A = L U
-// with P given by perm[] as well as by ipiv[] (L is stored in final row
-// order).  Outputs: L and U in place, ipiv (absolute row indices, as ZGETRF),
-// perm (ipiv + n), info and the inverses of the 64 x 64 diagonal blocks of L;
-// k_blkinv then inverts those of U.
+// with P given by ipiv[].  Outputs: L and U in place, ipiv (absolute row
+// indices, as ZGETRF), the pivot mode word and records (common.h), info and
+// the inverses of the 64 x 64 diagonal blocks of L; k_blkinv then inverts
+// those of U.  With left = 1 (pivot mode 0, the LAPACK form) L is in final
+// row order and perm (ipiv + n, P as a row list) is stored too; with left = 0
+// the columns left of each block keep their rows (mode 1).
 #include "common.h"
 
 // 16 x 16 x 4 MFMA: D[m][n] += sum_k A[m][k] B[k][n]; lane l supplies
@@ -545,21 +547,24 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
     }
   }  // blocks
   // ---- final row permutation: row r of L U is row perm[r] of the input,
-  // composed in LDS (s_slot, idle now) by one thread, then stored
-  int *perm = ipiv + n;
+  // composed in LDS (s_slot, idle now) by one thread, then stored; only the
+  // LAPACK form (pivot mode 0) has it: the mode-1 solves use the block records
   static_assert(sizeof(s_u) >= 1280 * sizeof(int), "the pivots fit in s_u");
-  int *s_ip = reinterpret_cast<int *>(s_u);   // the pivots, read by one thread
-  __syncthreads();
-  for (int r = t; r < n; r += NT) { s_slot[r] = r; s_ip[r] = ipiv[r]; }
-  __syncthreads();
-  if (t == 0) {
-    for (int c = 0; c < n; ++c) {
-      const int p = s_ip[c];
-      if (p != c) { const int a = s_slot[c]; s_slot[c] = s_slot[p]; s_slot[p] = a; }
+  if (left) {
+    int *perm = ipiv + n;
+    int *s_ip = reinterpret_cast<int *>(s_u);   // the pivots, read by one thread
+    __syncthreads();
+    for (int r = t; r < n; r += NT) { s_slot[r] = r; s_ip[r] = ipiv[r]; }
+    __syncthreads();
+    if (t == 0) {
+      for (int c = 0; c < n; ++c) {
+        const int p = s_ip[c];
+        if (p != c) { const int a = s_slot[c]; s_slot[c] = s_slot[p]; s_slot[p] = a; }
+      }
     }
+    __syncthreads();
+    for (int r = t; r < n; r += NT) perm[r] = s_slot[r];
   }
-  __syncthreads();
-  for (int r = t; r < n; r += NT) perm[r] = s_slot[r];
   if (t == 0) { info[pi] = myinfo; ipiv[2 * n] = left ? 0 : 1; }
 }
 
