@@ -130,7 +130,7 @@ struct Batch {
   long long *doff = nullptr;
   zd *lead_vec = nullptr;
   int *pivs = nullptr, *info = nullptr;
-  int *order = nullptr;   // points by decreasing order npad (work-group -> point in k_lu, k_invit)
+  int *order = nullptr;   // point indexes by decreasing npad (work-group -> point in k_lu, k_invit)
   KSState *state = nullptr;
   int npadmax = 0, nymax = 0, nvecmax = 0;
   size_t nscales = 0;

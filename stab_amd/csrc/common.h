@@ -92,8 +92,8 @@ struct PointDev {
 };
 
 // Pivot storage of one point (ints at piv_off): ipiv[npad] (absolute rows,
-// as ZGETRF), perm[npad] (row r of L U is row perm[r] of the input), then a
-// mode word and one record per 64-column block of the LU:
+// as ZGETRF), perm[npad] (mode 0 only: row r of L U is row perm[r] of the
+// input), then a mode word and one record per 64-column block of the LU:
 //   mode 0: every interchange has also been applied to the columns left of
 //           its block (L in final row order, the LAPACK form);
 //   mode 1: it has not: the L columns of block b keep the row order reached

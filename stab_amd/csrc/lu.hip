@@ -8,8 +8,9 @@
 // touch memory; each panel then updates the rest of its block column (all
 // rows) in strips of SW columns, applying its interchanges through LDS and
 // forming its rows of U as inv(L11) * A12.  When the block column is done:
-//   * its net row permutation is applied to the columns left of it and to the
-//     trailing columns (one pass each),
+//   * its net row permutation is applied once to the trailing columns and,
+//     for the LAPACK form only (left = 1), to the columns left of it;
+//     otherwise it is stored as the block's interchange record for the solves,
 //   * X = inv(L11) is formed once (four lanes per column, shuffles) and
 //     stored as the block's inverse for the triangular solves, and U12 = X A12
 //     runs on the matrix cores, and
