@@ -284,34 +284,31 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
           A[(size_t)(c0 + cc) * lda + k0 + i] = u;
         }
         __syncthreads();
-        // A22 strip -= L21 U12: R rows x G columns of independent accumulators
+        // A22 strip -= L21 U12, one row slot at a time with G columns of
+        // independent accumulators; a wave skips the slots in which none of
+        // its 64 rows is below the panel and inside the matrix (the third
+        // slot of matrices of order <= 2 NT, for instance)
         constexpr int G = (R >= 2) ? 2 : 4;
+        const int wrow = t - lane;
 #pragma unroll 1
         for (int cg = 0; cg < sw; cg += G) {
-          C acc[R][G];
 #pragma unroll
           for (int j = 0; j < R; ++j) {
+            const int w0 = wrow + j * NT;
+            if (w0 >= n || w0 + 63 < k0 + NB) continue;
             const int r = t + j * NT;
             const bool live = (r >= k0 + NB && r < n);
+            C acc[G];
 #pragma unroll
-            for (int q = 0; q < G; ++q) acc[j][q] = live ? A[(size_t)(c0 + cg + q) * lda + r] : czero<T>();
-          }
+            for (int q = 0; q < G; ++q) acc[q] = live ? A[(size_t)(c0 + cg + q) * lda + r] : czero<T>();
 #pragma unroll
-          for (int k = 0; k < NB; ++k) {
-            C u[G];
+            for (int k = 0; k < NB; ++k) {
 #pragma unroll
-            for (int q = 0; q < G; ++q) u[q] = s_u12[k * SW + cg + q];
+              for (int q = 0; q < G; ++q) acc[q] = cfms(acc[q], Pn[j][k], s_u12[k * SW + cg + q]);
+            }
+            if (live) {
 #pragma unroll
-            for (int j = 0; j < R; ++j)
-#pragma unroll
-              for (int q = 0; q < G; ++q) acc[j][q] = cfms(acc[j][q], Pn[j][k], u[q]);
-          }
-#pragma unroll
-          for (int j = 0; j < R; ++j) {
-            const int r = t + j * NT;
-            if (r >= k0 + NB && r < n) {
-#pragma unroll
-              for (int q = 0; q < G; ++q) A[(size_t)(c0 + cg + q) * lda + r] = acc[j][q];
+              for (int q = 0; q < G; ++q) A[(size_t)(c0 + cg + q) * lda + r] = acc[q];
             }
           }
         }
