@@ -157,7 +157,8 @@ int stab_pencil(stab_ctx *ctx, const stab_point *pt, int which, stab_z *out);
  *   evec   NULL, or for point p a block of nvec_p*STAB_MAX_NEV values at offset
  *          sum_{q<p} nvec_q*STAB_MAX_NEV, nvec = 5 ny (temporal) or 10 ny
  *          (spatial: z = [phi; alpha phi], as the companion of
- *          spatial.f90:1007-1016 returns), column k = pair k
+ *          spatial.f90:1007-1016 returns), column k = pair k; columns
+ *          nev..STAB_MAX_NEV-1 (and eval/resid entries past nev) are zero
  *   resid  npts*STAB_MAX_NEV relative residuals
  *          ||(A - w B) x|| / ((||A|| + |w| ||B||) ||x||) in FP64 (Frobenius-type
  *          norms of the pencil's coefficient operators), or NULL
@@ -192,7 +193,11 @@ int stab_lu_factors(stab_ctx *ctx, const stab_point *pt, int mode, stab_z *lu, i
 size_t stab_evec_bytes(int npts, const stab_point *pts);
 
 /* Leading eigenmode per point: the least stable discrete mode (largest
- * Im omega for temporal points, smallest Im alpha for spatial points).  The
+ * Im omega for temporal points, smallest Im alpha for spatial points) within
+ * the window the method searches -- the pt.nev pairs nearest pt.shift
+ * (STAB_LEAD_KRYLOV) or the pairs the iteration reaches from pt.shift
+ * (STAB_LEAD_INVIT) -- not over the whole spectrum.  Columns and entries
+ * past what a point returns are zero.  The
  * reference leaves this choice to the user, who picks a mode from the sorted
  * spectrum (temporal.f90:844-879) with getevec -v and a guess
  * (getevec.f90:160-173, thesis/TStest/run.sh:13-17).

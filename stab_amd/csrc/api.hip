@@ -110,6 +110,15 @@ int check_point(stab_ctx *c, const stab_point &p, int idx) {
   if (p.nev < 1 || p.nev > STAB_MAX_NEV) return fail(c, STAB_E_ARG, "point %d: nev %d outside 1..%d", idx, p.nev, STAB_MAX_NEV);
   if (p.Ma == 0.0) return fail(c, STAB_E_ARG, "point %d: Ma = 0 (pressure scale 1/(gamma Ma^2))", idx);
   if (p.mattyp == 1 && !(p.Te > 0.0)) return fail(c, STAB_E_ARG, "point %d: Sutherland law needs Te > 0", idx);
+  {
+    const double f[] = {p.Ma, p.Re, p.Pr, p.Te, p.yi, p.ymax, p.alpha.re, p.alpha.im, p.beta.re, p.beta.im,
+                        p.omega.re, p.omega.im, p.shift.re, p.shift.im};
+    const char *nm[] = {"Ma", "Re", "Pr", "Te", "yi", "ymax", "alpha.re", "alpha.im", "beta.re", "beta.im",
+                        "omega.re", "omega.im", "shift.re", "shift.im"};
+    for (int k = 0; k < (int)(sizeof(f) / sizeof(f[0])); ++k)
+      if (!std::isfinite(f[k])) return fail(c, STAB_E_ARG, "point %d: %s is not finite", idx, nm[k]);
+  }
+  if (!(p.Pr > 0.0)) return fail(c, STAB_E_ARG, "point %d: Pr must be positive", idx);
   if (!c->profiles.count(p.profile)) return fail(c, STAB_E_NOPROFILE, "point %d: profile slot %d not set", idx, p.profile);
   return STAB_OK;
 }
@@ -540,6 +549,8 @@ const char *stab_last_error(const stab_ctx *c) { return c ? c->err.c_str() : "no
 int stab_profile_set(stab_ctx *c, int slot, int nym, const double *y, const double *vt) {
   if (!c) return STAB_E_ARG;
   if (nym < 4 || !y || !vt) return fail(c, STAB_E_ARG, "profile needs at least 4 rows (got %d)", nym);
+  for (int j = 0; j < nym; ++j)
+    if (!std::isfinite(y[j])) return fail(c, STAB_E_ARG, "profile y of row %d is not finite", j);
   for (int j = 1; j < nym; ++j)
     if (!(y[j] > y[j - 1])) return fail(c, STAB_E_ARG, "profile y must increase strictly (row %d)", j);
   for (int k = 0; k < 5 * nym; ++k)
@@ -830,6 +841,12 @@ static int solve_impl(stab_ctx *c, int npts, const stab_point *pts, int mode, st
     }
     info[i] = inf[i] ? STAB_INFO_SINGULAR : (hs[i].done == 1 ? STAB_INFO_OK : STAB_INFO_NOCONV);
     if (irbad) (*irbad)[i] = hs[i].irbad || inf[i];
+    if (evec) {   // columns past nev are not written by the device: return zeros, not stale pool data
+      const PointDev &P = B.h[i];
+      const size_t col0 = (size_t)P.out_off + (size_t)P.nvec * (size_t)pts[i].nev;
+      const size_t col1 = (size_t)P.out_off + (size_t)P.nvec * STAB_MAX_NEV;
+      if (col1 > col0) memset(evec + col0, 0, sizeof(stab_z) * (col1 - col0));
+    }
     steps[i] = hs[i].steps;
   }
   if (stats) fill_stats(stats, npts, pts, B, ri, mode == STAB_MODE_MIXED, steps);

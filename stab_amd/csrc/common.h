@@ -52,6 +52,16 @@ template <typename T> __host__ __device__ inline cx<T> cfmac(cx<T> a, cx<T> b, c
   return a;
 }
 template <typename T> __host__ __device__ inline cx<T> czero() { return mk<T>(0, 0); }
+// A normalisation divisor is used only when it is finite and nonzero; a vector
+// whose largest entry is 0, Inf or NaN is left as it is (its residual then
+// reports the failure).
+// Index found by an arg-max over n entries, or 0 when no entry compared
+// greater than the start value (all NaN): always a valid index.
+__host__ __device__ inline int clamp_index(int bi, int n) { return (bi >= 0 && bi < n) ? bi : 0; }
+template <typename T> __host__ __device__ inline bool scale_ok(cx<T> a) {
+  const T m = fmax(fabs(a.x), fabs(a.y));
+  return m > (T)0 && (a.x - a.x) == (T)0 && (a.y - a.y) == (T)0;   // x - x is NaN for Inf and NaN
+}
 
 template <typename To, typename Ti> __host__ __device__ inline cx<To> cvt(cx<Ti> a) { return mk<To>((To)a.x, (To)a.y); }
 

@@ -379,6 +379,21 @@ __device__ inline double hash01(unsigned a, unsigned b) {
   return (double)h * (1.0 / 4294967296.0) - 0.5;
 }
 
+// Seed of a point's start vector from the point's own parameters, so that its
+// iterates do not depend on where it sits in the batch.
+__device__ inline unsigned point_seed(const PointDev &P) {
+  const double f[] = {P.alpha.x, P.alpha.y, P.beta.x, P.beta.y, P.omega.x, P.omega.y, P.shift.x, P.shift.y,
+                      P.Re, P.Ma, P.Pr, P.Te, P.yi, P.ymax};
+  unsigned long long h = 0xCBF29CE484222325ull ^ (unsigned long long)(P.ny * 4 + P.itype);
+#pragma unroll
+  for (int k = 0; k < (int)(sizeof(f) / sizeof(f[0])); ++k) {
+    h ^= (unsigned long long)__double_as_longlong(f[k]);
+    h *= 0x100000001B3ull;
+    h ^= h >> 29;
+  }
+  return (unsigned)(h ^ (h >> 32));
+}
+
 // ---------------------------------------------------------------------------
 // Arnoldi expansion from column j0 to m (CGS2).  V column-major, ld = ldv.
 // ---------------------------------------------------------------------------
@@ -407,7 +422,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FULL ? 2 : 3
   bool allok = true;
   if (start) {
     // v0 = T r / ||T r||, r pseudo-random
-    for (int r = t; r < nv; r += NT) V[r] = mk(hash01(blockIdx.x, 2 * r), hash01(blockIdx.x, 2 * r + 1));
+    const unsigned seed = point_seed(P);
+    for (int r = t; r < nv; r += NT) V[r] = mk(hash01(seed, 2 * r), hash01(seed, 2 * r + 1));
     __threadfence_block();
     __syncthreads();
     allok &= apply_op<T, NB, NT, FULL>(P, A, ipiv, Lb, Ub, dr, dc, coef, ops, ir, V, L);
@@ -820,12 +836,13 @@ __global__ __launch_bounds__(NT) void k_final(const PointDev *__restrict__ pts, 
     if (t == 0) {
       for (int k = 1; k < NT; ++k)
         if (s_mx[k] > bm || (s_mx[k] == bm && s_ix[k] < bi)) { bm = s_mx[k]; bi = s_ix[k]; }
-      s_ix[0] = bi;
+      s_ix[0] = clamp_index(bi, n);   // all-NaN vector: no entry compares greater
     }
     __syncthreads();
     const zd scale = L.z[s_ix[0]];
     __syncthreads();
-    for (int r = t; r < nv; r += NT) L.z[r] = cdiv(L.z[r], scale);
+    if (scale_ok(scale))
+      for (int r = t; r < nv; r += NT) L.z[r] = cdiv(L.z[r], scale);
     __syncthreads();
     {
       const double ff = farfield_ratio<NT>(P, L.z, L.red);
@@ -978,7 +995,8 @@ __global__ __launch_bounds__(NT) void k_invit(const PointDev *__restrict__ pts, 
   __shared__ int s_ix[NT];
 
   // ---- start: x = L(sigma)^-1 L'(sigma) r
-  for (int r = t; r < n; r += NT) x[r] = mk(hash01(pi, 2 * r), hash01(pi, 2 * r + 1));
+  const unsigned seed = point_seed(P);
+  for (int r = t; r < n; r += NT) x[r] = mk(hash01(seed, 2 * r), hash01(seed, 2 * r + 1));
   __syncthreads();
   if (temporal) {
     pencil_apply<NT>(P, coef, ops, FORM_BM, sg, x, L.rhs, L.X1, L.X2, L.red);
@@ -1079,12 +1097,13 @@ __global__ __launch_bounds__(NT) void k_invit(const PointDev *__restrict__ pts, 
   if (t == 0) {
     for (int k = 1; k < NT; ++k)
       if (s_mx[k] > bm || (s_mx[k] == bm && s_ix[k] < bi)) { bm = s_mx[k]; bi = s_ix[k]; }
-    s_ix[0] = bi;
+    s_ix[0] = clamp_index(bi, n);     // all-NaN vector: no entry compares greater
   }
   __syncthreads();
   const zd scale = x[s_ix[0]];
   __syncthreads();
-  for (int r = t; r < n; r += NT) x[r] = cdiv(x[r], scale);
+  if (scale_ok(scale))
+    for (int r = t; r < n; r += NT) x[r] = cdiv(x[r], scale);
   __syncthreads();
   const double ff = farfield_ratio<NT>(P, x, L.red);
   zd *out = evec + P.out_off;
@@ -1329,12 +1348,12 @@ __global__ __launch_bounds__(NT) void k_invit2(const PointDev *__restrict__ pts,
     if (t == 0) {
       for (int j = 1; j < NT; ++j)
         if (s_mx[j] > bm || (s_mx[j] == bm && s_ix[j] < bi)) { bm = s_mx[j]; bi = s_ix[j]; }
-      s_ix[0] = bi;
+      s_ix[0] = clamp_index(bi, n);   // all-NaN vector: no entry compares greater
     }
     __syncthreads();
     const zd scale = x[s_ix[0]];
     __syncthreads();
-    if (abs2(scale) > 0.0)
+    if (scale_ok(scale))
       for (int r = t; r < n; r += NT) x[r] = cdiv(x[r], scale);
     __syncthreads();
     const double ff = farfield_ratio<NT>(P, x, L.red);

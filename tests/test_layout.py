@@ -1,7 +1,11 @@
 """CPU checks of the device-buffer layouts: every index a kernel forms stays
 inside the allocation the host makes for it (restated from the kernels'
 index expressions), for all supported orders."""
+import os
+
 import pytest
+
+from conftest import ROOT
 
 BS = 64          # SOLVE_BS
 LU_ALIGN = 32
@@ -54,3 +58,44 @@ def test_krylov_state_fits():
     for m in range(2, MMAX + 1):
         j = m - 1
         assert j * (MMAX + 1) + (j + 1) < (MMAX + 1) * MMAX
+
+
+def test_normalisation_guards_on_host(tmp_path):
+    """The arg-max clamp and the divisor guard of the getevec normalisation
+    (stab_amd/csrc/common.h), compiled for the host and run on NaN, Inf, zero
+    and ordinary inputs."""
+    import shutil
+    import subprocess
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not os.path.exists(hipcc):
+        pytest.skip("hipcc not available")
+    src = tmp_path / "guards.cpp"
+    src.write_text(r'''
+#include "common.h"
+#include <cmath>
+#include <cstdio>
+int main() {
+  int bad = 0;
+  bad += clamp_index(0x7fffffff, 10) != 0;
+  bad += clamp_index(-1, 10) != 0;
+  bad += clamp_index(9, 10) != 9;
+  bad += clamp_index(10, 10) != 0;
+  const double nan = std::nan(""), inf = INFINITY;
+  bad += scale_ok(mk(nan, 0.0));
+  bad += scale_ok(mk(0.0, inf));
+  bad += scale_ok(mk(0.0, 0.0));
+  bad += !scale_ok(mk(1e-300, 0.0));
+  bad += !scale_ok(mk(-3.0, 4.0));
+  bad += !scale_ok(mk(1e308, -1e308));
+  std::printf("%d\n", bad);
+  return 0;
+}
+''')
+    exe = tmp_path / "guards"
+    r = subprocess.run([hipcc, "-x", "hip", "--cuda-host-only", "-O1", "-I",
+                        os.path.join(ROOT, "stab_amd", "csrc"), str(src), "-o", str(exe)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]
+    out = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert out.returncode == 0
+    assert out.stdout.strip() == "0"
