@@ -92,42 +92,81 @@ def load_traffic(kernel: str, key: str):
     return None
 
 
-def cpu_worker(args):
-    """One point of the CPU baseline: the oracle's reference method
-    (ZGESV + ZGEEV) timed, plus ZGGEV on the same pencil for the check."""
-    os.environ["OPENBLAS_NUM_THREADS"] = "1"
-    tab, a, b, r, n = args
+def _cpu_solve(tab, a, b, r, n):
+    """The oracle's reference method for one point (pencil assembly, ZGESV, ZGEEV,
+    sort and scaling; temporal.f90:600-879), the part that is timed."""
     from oracle import stab_oracle as so
     txt = (f"1\n288.15\n0.5 {float(r)!r} 0.72\n{int(n)} {2.0 / 46.0!r} 50\n1\n1\n0 0 0 0\n1\n"
            f"{float(a)!r} 0\n{float(b)!r} 0\n0\n")
     p = so.parse_input(txt)
-    t0 = time.perf_counter()
     pb = so.setup(p, tab[:, 0], tab[:, 1:6].copy(), spatial=False)
     A0, B0 = so.temporal_pencil(pb)
-    w, v = so.temporal_solve_reference(A0, B0)
-    dt = time.perf_counter() - t0
-    wq, _ = so.qz(A0, B0, right=False)
-    wq = wq[np.isfinite(wq)]
-    return dt, wq
+    w, _ = so.temporal_solve_reference(A0, B0)
+    return A0, B0, w
 
 
-def cpu_baseline(tab, al, be, re, ny, chosen_evals, sample, workers):
-    import multiprocessing as mp
-    idx = list(range(sample))
-    jobs = [(tab, al[i], be[i], re[i], ny[i]) for i in idx]
+def cpu_worker(args):
+    """One sample point: the reference method timed alone (nothing else in the
+    clock), its eigenvalues returned for the check."""
+    tab, a, b, r, n = args
     t0 = time.perf_counter()
+    _, _, w = _cpu_solve(tab, a, b, r, n)
+    return time.perf_counter() - t0, w[np.isfinite(w)]
+
+
+def cpu_latency(args):
+    """Single-point latency at order N of the reference method (ZGESV + ZGEEV) or
+    of the QZ judge (ZGGEV, eigenvalues and right vectors) on one core."""
+    tab, n, method = args
+    from oracle import stab_oracle as so
+    A0, B0, _ = _cpu_solve(tab, 0.1, 0.05, 1000.0, n)          # warm caches and imports at this size
+    t0 = time.perf_counter()
+    if method == "reference":
+        _cpu_solve(tab, 0.1, 0.05, 1000.0, n)
+    else:
+        so.qz(A0, B0, right=True)
+    return n, method, time.perf_counter() - t0
+
+
+def _cpu_warm(_):
+    import scipy.linalg  # noqa: F401
+    from oracle import stab_oracle  # noqa: F401
+    return os.getpid()
+
+
+def cpu_workers() -> int:
+    """Host cores the CPU baseline uses: the process's CPU affinity, at most 16
+    (the host share of one GPU)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_baseline(tab, al, be, re, ny, workers, per_worker=4, latency=True):
+    """Reference method on the host cores: a warmed pool of `workers`
+    single-threaded processes (OPENBLAS_NUM_THREADS=1), per_worker points each
+    from the head of the sweep, each solve timed by itself.  Throughput =
+    workers / mean per-point solve time (every core busy for the whole run)."""
+    import multiprocessing as mp
+    os.environ["OPENBLAS_NUM_THREADS"] = "1"      # inherited by the spawned workers before numpy loads
+    nsample = per_worker * workers
+    jobs = [(tab, al[i], be[i], re[i], ny[i]) for i in range(nsample)]
     with mp.get_context("spawn").Pool(workers) as pool:
-        out = pool.map(cpu_worker, jobs)
-    wall = time.perf_counter() - t0
-    match = total = 0
-    for (dt, wq), i in zip(out, idx):
-        ev = chosen_evals[i]
-        if not np.isfinite(ev) or ev == 0:
-            continue
-        total += 1
-        match += int(np.min(np.abs(wq - ev)) <= 1e-8 * abs(ev))
-    per_point = sum(o[0] for o in out) / len(out)
-    return dict(value=len(out) / wall, per_point_s=per_point, wall_s=wall, match=match, total=total)
+        pool.map(_cpu_warm, range(2 * workers), chunksize=1)
+        t0 = time.perf_counter()
+        out = pool.map(cpu_worker, jobs, chunksize=1)
+        wall = time.perf_counter() - t0
+        lat = {}
+        if latency:
+            for n, method, dt in pool.map(cpu_latency, [(tab, n, m) for n in (64, 96, 128, 256)
+                                                        for m in ("reference", "qz")], chunksize=1):
+                lat[f"{method}_N{n}_s"] = dt
+    per_point = np.array([o[0] for o in out])
+    return dict(points_per_s=workers / per_point.mean(), wall_points_per_s=nsample / wall,
+                per_point_s=float(per_point.mean()), sample=nsample, spectra=[o[1] for o in out], latency=lat,
+                sample_idx=list(range(nsample)))
 
 
 def main():
@@ -139,7 +178,7 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--nev", type=int, default=8)
     ap.add_argument("--maxit", type=int, default=20)
-    ap.add_argument("--cpu-sample", type=int, default=16)
+    ap.add_argument("--cpu-per-worker", type=int, default=4, help="CPU baseline points per host core")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["plain", "chosen"], default=None, help="time one path only (profiling)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -294,16 +333,25 @@ def main():
         }
         if not args.no_cpu:
             print("# bench: CPU baseline sample", file=sys.stderr, flush=True)
-            workers = max(1, min(16, os.cpu_count() or 1, args.cpu_sample))
+            workers = cpu_workers()
+            cb = cpu_baseline(tab, al, be, re, ny, workers, args.cpu_per_worker)
             ev = np.where(delivered, cres["evals"], np.nan)
-            cb = cpu_baseline(tab, al, be, re, ny, ev, args.cpu_sample, workers)
+            match = total = 0
+            for i, wq in zip(cb["sample_idx"], cb["spectra"]):
+                if np.isfinite(ev[i]) and ev[i] != 0:
+                    total += 1
+                    match += int(np.min(np.abs(wq - ev[i])) <= 1e-8 * abs(ev[i]))
             out["cpu_baseline"] = {
-                "value": cb["value"], "unit": "leading eigenmodes/s", "cores": workers, "kind": "port",
-                "sample": f"first {args.cpu_sample} points of the same seeded sweep, reference method (ZGESV+ZGEEV, "
-                          f"whole spectrum, one leading mode per point), one point per process, "
-                          f"OPENBLAS_NUM_THREADS=1; {cb['per_point_s']:.2f} s per point on one core",
-                "qz_match": f"{cb['match']}/{cb['total']} chosen-path leading eigenvalues within 1e-8 of a ZGGEV "
-                            f"eigenvalue"}
+                "value": cb["points_per_s"], "unit": "leading eigenmodes/s", "cores": workers, "kind": "port",
+                "sample": f"first {cb['sample']} points of the same seeded sweep ({args.cpu_per_worker} per core), "
+                          f"the oracle's reference method (pencil, ZGESV, ZGEEV: the whole spectrum, whence the "
+                          f"leading mode) on {workers} single-threaded processes (OPENBLAS_NUM_THREADS=1) of "
+                          f"{os.cpu_count()} logical CPUs; {cb['per_point_s']:.3f} s per point; value = cores / "
+                          f"mean per-point solve time",
+                "wall_points_per_s": cb["wall_points_per_s"],
+                "latency_one_core_s": cb["latency"],
+                "check": f"{match}/{total} sampled chosen-path leading eigenvalues within 1e-8 of an eigenvalue of the "
+                         f"reference method"}
         else:
             out["cpu_baseline"] = None
     else:
