@@ -227,6 +227,24 @@ int stab_solve_leading(stab_ctx *ctx, int npts, const stab_point *pts, int mode,
                        const stab_z *fallback_shift, stab_z *eval, stab_z *evec, double *resid,
                        double *farfield, int32_t *iters, int32_t *info, stab_stats *stats);
 
+/* Allocation-free variant of stab_solve_leading.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory during the
+ * call (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same arguments, results and kernels as
+ * stab_solve_leading; every device buffer of the call is carved from one
+ * static device array of STAB_NOALLOC_SCRATCH_BYTES per GPU (held until the
+ * process ends), so the call makes no hipMalloc/hipFree.  Points whose
+ * worst-case need (first pass plus an FP64 Krylov-Schur fallback of every
+ * point) exceeds that size are refused with STAB_E_ARG before any work --
+ * roughly five N = 128 points or twenty N = 64 points per call.  All calls on
+ * one GPU share the array and leave their data in it: calls that overlap in
+ * time on the same GPU (two host threads, two contexts) race.  Profiles
+ * must already be set (stab_profile_set allocates). */
+#define STAB_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int stab_solve_leading_noalloc(stab_ctx *ctx, int npts, const stab_point *pts, int mode, int method, int maxit,
+                               const stab_z *fallback_shift, stab_z *eval, stab_z *evec, double *resid,
+                               double *farfield, int32_t *iters, int32_t *info, stab_stats *stats);
+
 /* ---------------------------------------------------------------------------
  * Shift predictor.  No counterpart in the reference, whose user supplies the
  * eigenvalue guess by hand (getevec -v, thesis/TStest/run.sh:13-17): a tanh

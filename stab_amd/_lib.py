@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(_HERE, "libstab_hip.so")
 
 STAB_OK = 0
 STAB_MAX_NEV = 16
+NOALLOC_SCRATCH_BYTES = 64 << 20
 MODE_PLAIN_F64 = 0
 MODE_MIXED = 1
 MAT_A0, MAT_B0, MAT_C0, MAT_C1, MAT_C2, MAT_SHIFTED = range(6)
@@ -22,6 +23,7 @@ FARFIELD_MAX, CONTINUUM_BAND = 0.1, 0.01
 
 EXPORTS = ("stab_ctx_create", "stab_ctx_destroy", "stab_last_error", "stab_profile_set", "stab_grid",
            "stab_pencil", "stab_solve", "stab_evec_bytes", "stab_shifted_solve", "stab_lu_factors", "stab_copy_probe", "stab_solve_leading",
+           "stab_solve_leading_noalloc",
            "stab_mlp_create", "stab_mlp_destroy", "stab_mlp_nparams", "stab_mlp_get_params", "stab_mlp_set_params",
            "stab_mlp_set_scaling", "stab_mlp_predict", "stab_mlp_loss_grad", "stab_mlp_train")
 
@@ -74,6 +76,7 @@ def load():
     lib.stab_copy_probe.argtypes = [vp, C.c_int64, C.c_int, dp]
     lib.stab_solve_leading.argtypes = [vp, C.c_int, C.POINTER(Point), C.c_int, C.c_int, C.c_int, C.POINTER(Z),
                                        C.POINTER(Z), C.POINTER(Z), dp, dp, ip, ip, C.POINTER(Stats)]
+    lib.stab_solve_leading_noalloc.argtypes = lib.stab_solve_leading.argtypes
     fp = C.POINTER(C.c_float)
     lib.stab_mlp_create.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, C.POINTER(vp)]
     lib.stab_mlp_destroy.argtypes = [vp]

@@ -14,6 +14,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -31,6 +32,17 @@ struct Profile {
 
 }  // namespace
 
+// Fixed device memory handed out by buf_get while an allocation-free entry
+// point runs: named slots carved from the front of a caller-independent
+// block (the library's static scratch array), a slot reused when a later
+// request of the same name fits in it.
+struct Arena {
+  char *base = nullptr;
+  size_t cap = 0, top = 0;
+  bool on = false;
+  std::map<std::string, DevBuf> slots;
+};
+
 struct stab_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -38,6 +50,7 @@ struct stab_ctx {
   std::map<int, Profile> profiles;
   std::map<std::string, DevBuf> bufs;
   hipEvent_t ev[8];
+  Arena arena;
 };
 
 namespace {
@@ -59,8 +72,26 @@ int fail(stab_ctx *c, int code, const char *fmt, ...) {
       return fail(ctx, STAB_E_HIP, "%s failed: %s", #x, hipGetErrorString(e_));       \
   } while (0)
 
-// Grow-only named device buffer.
+constexpr size_t ARENA_ALIGN = 256;
+inline size_t arena_round(size_t b) { return (std::max(b, (size_t)ARENA_ALIGN) + ARENA_ALIGN - 1) / ARENA_ALIGN * ARENA_ALIGN; }
+
+// Grow-only named device buffer (from the arena while one is active).
 int buf_get(stab_ctx *c, const char *name, size_t bytes, void **out) {
+  if (c->arena.on) {
+    Arena &a = c->arena;
+    DevBuf &b = a.slots[name];
+    if (b.bytes < bytes) {
+      const size_t want = arena_round(bytes);
+      if (a.top + want > a.cap)
+        return fail(c, STAB_E_NOMEM, "allocation-free call: %s needs %zu more bytes than the %zu-byte scratch holds",
+                    name, a.top + want - a.cap, a.cap);
+      b.p = a.base + a.top;
+      b.bytes = want;
+      a.top += want;
+    }
+    *out = b.p;
+    return STAB_OK;
+  }
   DevBuf &b = c->bufs[name];
   if (b.bytes < bytes) {
     if (b.p) (void)hipFree(b.p);
@@ -147,8 +178,48 @@ struct Batch {
   std::map<int, int> op_off;
 };
 
-// Build the device layout for points [0, npts).  elem = sizeof LU element.
-int build_batch(stab_ctx *c, int npts, const stab_point *pts, size_t elem, int m, Batch &B) {
+// Device buffers of one batch, in the order build_batch takes them.
+struct BatchSizes {
+  long long poff = 0, ooff = 0, coff = 0, moff = 0, voff = 0, pvoff = 0, boff = 0, soff = 0, eoff = 0, nvtot = 0;
+  size_t elem = 0;
+  int npts = 0;
+  // (name, bytes) of every buffer
+  std::vector<std::pair<const char *, size_t>> list() const {
+    const size_t n = (size_t)npts;
+    return {{"pts", sizeof(PointDev) * n},
+            {"prof", sizeof(double) * poff},
+            {"ops", sizeof(double) * ooff},
+            {"coef", sizeof(zd) * coff},
+            {"grid", sizeof(double) * 9 * 256 * n},
+            {"mats", elem * moff},
+            {"pivs", sizeof(int) * pvoff},
+            {"blks", elem * boff},
+            {"scales", sizeof(double) * soff},
+            {"norms", sizeof(double) * 4 * n},
+            {"V", sizeof(zd) * voff},
+            {"small", sizeof(zd) * SM_SIZE * n},
+            {"state", sizeof(KSState) * n},
+            {"info", sizeof(int) * n},
+            {"eval", sizeof(zd) * KS_NEVMAX * n},
+            {"resid", sizeof(double) * KS_NEVMAX * n},
+            {"evec", sizeof(zd) * (eoff > 0 ? eoff : 1)},
+            {"ff", sizeof(double) * KS_NEVMAX * n},
+            {"iters", sizeof(int) * n},
+            {"sel", sizeof(int) * n},
+            {"flag", sizeof(int) * n},
+            {"doff", sizeof(long long) * (n + 1)},
+            {"order", sizeof(int) * n},
+            {"lead_vec", sizeof(zd) * (nvtot > 0 ? nvtot : 1)}};
+  }
+  size_t arena_bytes() const {
+    size_t t = 0;
+    for (auto &e : list()) t += arena_round(e.second);
+    return t;
+  }
+};
+
+// Host-side layout of points [0, npts) (no device calls).  elem = sizeof LU element.
+void layout_batch(stab_ctx *c, int npts, const stab_point *pts, size_t elem, int m, Batch &B, BatchSizes &S) {
   B.h.resize(npts);
   // profiles used
   std::map<int, long long> prof_off;
@@ -210,41 +281,35 @@ int build_batch(stab_ctx *c, int npts, const stab_point *pts, size_t elem, int m
     B.nvecmax = std::max(B.nvecmax, std::max(d.nvec, d.npad));
   }
   B.nscales = (size_t)soff;
-  int rc;
-  void *p;
-#define GET(name, bytes, dst)                                  \
-  if ((rc = buf_get(c, name, (size_t)(bytes), &p)) != STAB_OK) \
-    return rc;                                                 \
-  dst = decltype(dst)(p);
-  GET("pts", sizeof(PointDev) * npts, B.pts);
-  GET("prof", sizeof(double) * poff, B.prof);
-  GET("ops", sizeof(double) * ooff, B.ops);
-  GET("coef", sizeof(zd) * coff, B.coef);
-  GET("grid", sizeof(double) * 9 * 256 * npts, B.grid);
-  GET("mats", elem * moff, B.mats);
-  GET("pivs", sizeof(int) * pvoff, B.pivs);
-  GET("blks", elem * boff, B.blks);
-  GET("scales", sizeof(double) * soff, B.scales);
-  GET("norms", sizeof(double) * 4 * npts, B.norms);
-  GET("V", sizeof(zd) * voff, B.V);
-  GET("small", sizeof(zd) * SM_SIZE * (size_t)npts, B.small);
-  GET("state", sizeof(KSState) * npts, B.state);
-  GET("info", sizeof(int) * npts, B.info);
-  GET("eval", sizeof(zd) * KS_NEVMAX * (size_t)npts, B.eval);
-  GET("resid", sizeof(double) * KS_NEVMAX * (size_t)npts, B.resid);
-  GET("evec", sizeof(zd) * (eoff > 0 ? eoff : 1), B.evec);
-  GET("ff", sizeof(double) * KS_NEVMAX * (size_t)npts, B.ff);
-  GET("iters", sizeof(int) * npts, B.iters);
-  GET("sel", sizeof(int) * npts, B.sel);
-  GET("flag", sizeof(int) * npts, B.flag);
-  GET("doff", sizeof(long long) * (npts + 1), B.doff);
-  GET("order", sizeof(int) * npts, B.order);
-  {
-    long long tot = 0;
-    for (int i = 0; i < npts; ++i) tot += B.h[i].nvec;
-    GET("lead_vec", sizeof(zd) * (tot > 0 ? tot : 1), B.lead_vec);
+  S.poff = poff; S.ooff = ooff; S.coff = coff; S.moff = moff; S.voff = voff; S.pvoff = pvoff; S.boff = boff;
+  S.soff = soff; S.eoff = eoff; S.elem = elem; S.npts = npts;
+  S.nvtot = 0;
+  for (int i = 0; i < npts; ++i) S.nvtot += B.h[i].nvec;
+}
+
+// Build the device layout for points [0, npts).  elem = sizeof LU element.
+int build_batch(stab_ctx *c, int npts, const stab_point *pts, size_t elem, int m, Batch &B) {
+  BatchSizes S;
+  layout_batch(c, npts, pts, elem, m, B, S);
+  std::vector<void *> ptr;
+  for (auto &e : S.list()) {
+    void *p;
+    int rc = buf_get(c, e.first, e.second, &p);
+    if (rc != STAB_OK) return rc;
+    ptr.push_back(p);
   }
-#undef GET
+  {
+    int k = 0;
+    B.pts = (PointDev *)ptr[k++]; B.prof = (double *)ptr[k++]; B.ops = (double *)ptr[k++];
+    B.coef = (zd *)ptr[k++]; B.grid = (double *)ptr[k++]; B.mats = ptr[k++]; B.pivs = (int *)ptr[k++];
+    B.blks = ptr[k++]; B.scales = (double *)ptr[k++]; B.norms = (double *)ptr[k++]; B.V = (zd *)ptr[k++];
+    B.small = (zd *)ptr[k++]; B.state = (KSState *)ptr[k++]; B.info = (int *)ptr[k++]; B.eval = (zd *)ptr[k++];
+    B.resid = (double *)ptr[k++]; B.evec = (zd *)ptr[k++]; B.ff = (double *)ptr[k++]; B.iters = (int *)ptr[k++];
+    B.sel = (int *)ptr[k++]; B.flag = (int *)ptr[k++]; B.doff = (long long *)ptr[k++]; B.order = (int *)ptr[k++];
+    B.lead_vec = (zd *)ptr[k++];
+  }
+  std::map<int, long long> prof_off;
+  for (int i = 0; i < npts; ++i) prof_off[pts[i].profile] = B.h[i].prof_off;
   HIPCHECK(c, hipMemcpyAsync(B.pts, B.h.data(), sizeof(PointDev) * npts, hipMemcpyHostToDevice, c->stream));
   // Largest matrices first: work-groups start roughly in index order, so the
   // batch then ends on its cheapest points instead of on a few large ones.
@@ -381,6 +446,10 @@ int env_int(const char *name, int dflt) {
   return v ? atoi(v) : dflt;
 }
 
+// Krylov-Schur basis size for a batch whose largest pair count is nevmax:
+// 24 (keep 12) up to 4 pairs, 40 (keep 20) above.
+int ks_basis(int nevmax) { return env_int("STAB_KS_M", nevmax <= 4 ? 24 : 40); }
+
 }  // namespace
 
 // Plain device copy, one 16-byte load and store per thread: the reference
@@ -510,6 +579,40 @@ int mlp_check_rows(stab_mlp *M, int n) {
   return STAB_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free entry point (stab_solve_leading_noalloc):
+// every device buffer of that call is carved from this array, which each GPU
+// the library runs on holds until the process ends.  Host code never passes
+// the array's name to a kernel or a copy: it is a host-side stand-in, not a
+// device address.  Each device's address comes from hipGetSymbolAddress,
+// taken once per device when the first context on it is created.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[STAB_NOALLOC_SCRATCH_BYTES];
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+char *g_scratch_ptr[SCRATCH_MAX_DEVICES];
+
+// With c->device current: resolve this device's scratch address once.
+int scratch_init(stab_ctx *c) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (c->device < 0 || c->device >= SCRATCH_MAX_DEVICES)
+    return fail(c, STAB_E_ARG, "device %d above the %d devices the scratch table holds", c->device,
+                SCRATCH_MAX_DEVICES);
+  if (g_scratch_ptr[c->device]) return STAB_OK;
+  void *ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    return fail(c, STAB_E_HIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+  g_scratch_ptr[c->device] = (char *)ptr;
+  return STAB_OK;
+}
+
+char *scratch_of(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return (device >= 0 && device < SCRATCH_MAX_DEVICES) ? g_scratch_ptr[device] : nullptr;
+}
+
 }  // namespace
 
 extern "C" {
@@ -528,6 +631,12 @@ int stab_ctx_create(stab_ctx **out, int device) {
     return STAB_E_HIP;
   }
   for (auto &e : c->ev) (void)hipEventCreate(&e);
+  if (scratch_init(c) != STAB_OK) {
+    for (auto &e : c->ev) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return STAB_E_HIP;
+  }
   *out = c;
   return STAB_OK;
 }
@@ -745,7 +854,7 @@ static int solve_run(stab_ctx *c, int npts, const stab_point *pts, int mode, Bat
   for (int i = 0; i < npts; ++i) nevmax = std::max(nevmax, pts[i].nev);
   // basis size scaled to the pairs wanted: 24 / keep 12 up to 4 pairs, 40 / 20 above
   KSParams kp;
-  kp.m = env_int("STAB_KS_M", nevmax <= 4 ? 24 : 40);
+  kp.m = ks_basis(nevmax);
   kp.keep = env_int("STAB_KS_KEEP", nevmax <= 4 ? 12 : 20);
   kp.maxcycles = env_int("STAB_KS_CYCLES", 10);
   kp.ir = (mode == STAB_MODE_MIXED) ? -env_int("STAB_IR_MAX", 12) : 0;   // adaptive refinement
@@ -1213,6 +1322,56 @@ int stab_solve_leading(stab_ctx *c, int npts, const stab_point *pts, int mode, i
     stats->restarts_max = std::max(stats->restarts_max, fst.restarts_max);
   }
   return STAB_OK;
+}
+
+int stab_solve_leading_noalloc(stab_ctx *c, int npts, const stab_point *pts, int mode, int method, int maxit,
+                               const stab_z *fallback_shift, stab_z *eval, stab_z *evec, double *resid,
+                               double *farfield, int32_t *iters, int32_t *info, stab_stats *stats) {
+  if (!c) return STAB_E_ARG;
+  if (npts <= 0 || !pts || !eval || !info) return fail(c, STAB_E_ARG, "npts/pts/eval/info");
+  if (mode != STAB_MODE_PLAIN_F64 && mode != STAB_MODE_MIXED) return fail(c, STAB_E_ARG, "unknown mode %d", mode);
+  for (int i = 0; i < npts; ++i) {
+    const int rc = check_point(c, pts[i], i);
+    if (rc) return rc;
+  }
+  char *base = scratch_of(c->device);
+  if (!base) return fail(c, STAB_E_HIP, "no scratch address for device %d", c->device);
+  // Worst case: the first pass plus an FP64 Krylov-Schur fallback of every point.
+  int nevmax = 0;
+  for (int i = 0; i < npts; ++i) nevmax = std::max(nevmax, pts[i].nev);
+  const size_t elem = (mode == STAB_MODE_MIXED) ? sizeof(zf) : sizeof(zd);
+  size_t need = 0;
+  {
+    Batch B;
+    BatchSizes S;
+    if (method == STAB_LEAD_KRYLOV) {
+      layout_batch(c, npts, pts, elem, ks_basis(nevmax), B, S);
+      need = S.arena_bytes();
+    } else {
+      layout_batch(c, npts, pts, elem, 1, B, S);
+      need = S.arena_bytes();
+      Batch B2;
+      BatchSizes S2;
+      layout_batch(c, npts, pts, sizeof(zd), ks_basis(nevmax), B2, S2);
+      need += S2.arena_bytes();
+    }
+  }
+  if (need > STAB_NOALLOC_SCRATCH_BYTES)
+    return fail(c, STAB_E_ARG,
+                "stab_solve_leading_noalloc: these %d points need up to %zu bytes of device scratch, above its "
+                "%zu-byte maximum (solve fewer or smaller points per call, or use stab_solve_leading)",
+                npts, need, (size_t)STAB_NOALLOC_SCRATCH_BYTES);
+  Arena &a = c->arena;
+  a.base = base;
+  a.cap = STAB_NOALLOC_SCRATCH_BYTES;
+  a.top = 0;
+  a.slots.clear();
+  a.on = true;
+  const int rc = stab_solve_leading(c, npts, pts, mode, method, maxit, fallback_shift, eval, evec, resid, farfield,
+                                    iters, info, stats);
+  a.on = false;
+  a.slots.clear();
+  return rc;
 }
 
 int stab_mlp_create(stab_ctx *c, int nin, int width, int depth, int nout, uint64_t seed, stab_mlp **out) {
