@@ -24,6 +24,8 @@ def hipcc() -> str:
 
 FORTRAN_DIR = os.path.join(HERE, "fortran")
 FORTRAN_BIN = os.path.join(HERE, "stab_gpu")
+GETEVEC_BIN = os.path.join(HERE, "getevec")
+EXAMPLE_BIN = os.path.join(HERE, "alpha_march")
 
 
 def fortran_compiler():
@@ -33,22 +35,32 @@ def fortran_compiler():
     return None
 
 
+def _fc(fc, work, sources, out, link_lib=True):
+    tmp = out + ".tmp"
+    cmd = [fc, "-O2", "-J", work, *sources]
+    if link_lib:
+        cmd += ["-L" + HERE, "-lstab_hip", "-Wl,-rpath,$ORIGIN"]
+    cmd += ["-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True, cwd=work)
+    if r.returncode != 0:
+        raise RuntimeError("Fortran build failed:\n" + " ".join(cmd) + "\n" + r.stdout + r.stderr)
+    os.replace(tmp, out)
+
+
 def build_fortran() -> str | None:
-    """Compile the Fortran host driver (ISO_C_BINDING onto the C ABI) next to
-    the library; returns its path, or None when no Fortran compiler exists."""
+    """Compile the Fortran layer (ISO_C_BINDING onto the C ABI) and the programs
+    over it next to the library: stab_gpu (the main program), getevec (host
+    only) and alpha_march (a caller written against the reference's interface).
+    Returns the stab_gpu path, or None when no Fortran compiler exists."""
     fc = fortran_compiler()
     if fc is None:
         return None
     work = os.path.join(ROOT, "_build", "fortran")
     os.makedirs(work, exist_ok=True)
-    tmp = FORTRAN_BIN + ".tmp"
-    cmd = [fc, "-O2", "-J", work, os.path.join(FORTRAN_DIR, "stab_hip_iface.f90"),
-           os.path.join(FORTRAN_DIR, "stab_gpu.f90"), "-L" + HERE, "-lstab_hip",
-           "-Wl,-rpath,$ORIGIN", "-o", tmp]
-    r = subprocess.run(cmd, capture_output=True, text=True, cwd=work)
-    if r.returncode != 0:
-        raise RuntimeError("Fortran build failed:\n" + r.stdout + r.stderr)
-    os.replace(tmp, FORTRAN_BIN)
+    layer = [os.path.join(FORTRAN_DIR, "stab_hip_iface.f90"), os.path.join(FORTRAN_DIR, "stab_path.f90")]
+    _fc(fc, work, layer + [os.path.join(FORTRAN_DIR, "stab_gpu.f90")], FORTRAN_BIN)
+    _fc(fc, work, layer + [os.path.join(FORTRAN_DIR, "examples", "alpha_march.f90")], EXAMPLE_BIN)
+    _fc(fc, work, [os.path.join(FORTRAN_DIR, "getevec.f90")], GETEVEC_BIN, link_lib=False)
     return FORTRAN_BIN
 
 

@@ -14,6 +14,10 @@ module stab_hip
   integer(c_int), parameter :: STAB_MAX_NEV = 16
   integer(c_int), parameter :: STAB_MODE_PLAIN_F64 = 0, STAB_MODE_MIXED = 1
   integer(c_int), parameter :: STAB_INFO_OK = 0, STAB_INFO_SINGULAR = 1, STAB_INFO_NOCONV = 2
+  integer(c_int), parameter :: STAB_INFO_FALLBACK = 3
+  integer(c_int), parameter :: STAB_LEAD_KRYLOV = 0, STAB_LEAD_INVIT = 1, STAB_TRACK = 2
+  integer(c_int), parameter :: STAB_MAT_A0 = 0, STAB_MAT_B0 = 1, STAB_MAT_C0 = 2, STAB_MAT_C1 = 3, &
+                               STAB_MAT_C2 = 4, STAB_MAT_SHIFTED = 5
 
   type, bind(C) :: stab_point
     integer(c_int32_t) :: itype, ny, mattyp, wallt, top, profile, nev, ider
@@ -62,6 +66,95 @@ module stab_hip
       real(c_double) :: resid(*)
       integer(c_int32_t) :: info(*)
       type(c_ptr), value :: stats
+    end function
+
+    integer(c_int) function stab_pencil(ctx, pt, which, out) bind(C, name="stab_pencil")
+      import :: c_ptr, c_int, c_double_complex, stab_point
+      type(c_ptr), value :: ctx
+      type(stab_point) :: pt
+      integer(c_int), value :: which
+      complex(c_double_complex) :: out(*)
+    end function
+
+    integer(c_int) function stab_shifted_solve(ctx, pt, mode, ir_steps, b, x, info) &
+        bind(C, name="stab_shifted_solve")
+      import :: c_ptr, c_int, c_int32_t, c_double_complex, stab_point
+      type(c_ptr), value :: ctx
+      type(stab_point) :: pt
+      integer(c_int), value :: mode, ir_steps
+      complex(c_double_complex) :: b(*), x(*)
+      integer(c_int32_t) :: info
+    end function
+
+    integer(c_int) function stab_lu_factors(ctx, pt, mode, lu, perm, scales, info) &
+        bind(C, name="stab_lu_factors")
+      import :: c_ptr, c_int, c_int32_t, c_double_complex, stab_point
+      type(c_ptr), value :: ctx
+      type(stab_point) :: pt
+      integer(c_int), value :: mode
+      complex(c_double_complex) :: lu(*)
+      integer(c_int32_t) :: perm(*)
+      type(c_ptr), value :: scales
+      integer(c_int32_t) :: info
+    end function
+
+    integer(c_size_t) function stab_evec_bytes(npts, pts) bind(C, name="stab_evec_bytes")
+      import :: c_int, c_size_t, stab_point
+      integer(c_int), value :: npts
+      type(stab_point) :: pts(*)
+    end function
+
+    integer(c_int) function stab_solve_leading(ctx, npts, pts, mode, method, maxit, fallback_shift, eval, evec, &
+                                               resid, farfield, iters, info, stats) &
+        bind(C, name="stab_solve_leading")
+      import :: c_ptr, c_int, c_int32_t, c_double, c_double_complex, stab_point
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: npts, mode, method, maxit
+      type(stab_point) :: pts(*)
+      type(c_ptr), value :: fallback_shift
+      complex(c_double_complex) :: eval(*)
+      type(c_ptr), value :: evec
+      real(c_double) :: resid(*), farfield(*)
+      integer(c_int32_t) :: iters(*), info(*)
+      type(c_ptr), value :: stats
+    end function
+
+    integer(c_int) function stab_mlp_create(ctx, nin, width, depth, nout, seed, mlp) bind(C, name="stab_mlp_create")
+      import :: c_ptr, c_int, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nin, width, depth, nout
+      integer(c_int64_t), value :: seed
+      type(c_ptr) :: mlp
+    end function
+
+    integer(c_int) function stab_mlp_destroy(mlp) bind(C, name="stab_mlp_destroy")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: mlp
+    end function
+
+    integer(c_int) function stab_mlp_nparams(mlp) bind(C, name="stab_mlp_nparams")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: mlp
+    end function
+
+    integer(c_int) function stab_mlp_set_params(mlp, params) bind(C, name="stab_mlp_set_params")
+      import :: c_ptr, c_int, c_float
+      type(c_ptr), value :: mlp
+      real(c_float) :: params(*)
+    end function
+
+    integer(c_int) function stab_mlp_set_scaling(mlp, in_shift, in_scale, out_shift, out_scale) &
+        bind(C, name="stab_mlp_set_scaling")
+      import :: c_ptr, c_int, c_float
+      type(c_ptr), value :: mlp
+      real(c_float) :: in_shift(*), in_scale(*), out_shift(*), out_scale(*)
+    end function
+
+    integer(c_int) function stab_mlp_predict(mlp, n, x, y) bind(C, name="stab_mlp_predict")
+      import :: c_ptr, c_int, c_float
+      type(c_ptr), value :: mlp
+      integer(c_int), value :: n
+      real(c_float) :: x(*), y(*)
     end function
   end interface
 
