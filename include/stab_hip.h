@@ -109,7 +109,9 @@ typedef struct {
 typedef struct stab_ctx stab_ctx;
 
 /* Solver statistics of the last stab_solve / stab_solve_leading call
- * (device-timed, ms). */
+ * (device-timed, ms).  When stab_solve_leading's first pass runs as a HIP graph
+ * (graph = 1 or 2) it is timed as a whole: ms_total holds it, and ms_prep,
+ * ms_assemble, ms_lu and ms_lu_kernel are 0 (set STAB_GRAPH=0 for them). */
 typedef struct {
   double ms_prep, ms_assemble, ms_lu, ms_eig, ms_total;
   int64_t lu_flops;       /* algorithmic (8/3) n^3 per point, summed */
@@ -121,6 +123,10 @@ typedef struct {
   int64_t op_applies;     /* shift-invert operator applications, summed over points */
   double solve_bytes;     /* algorithmic factor bytes read by those applications */
   double ms_lu_kernel;    /* device time of the LU kernel alone (ms_lu adds the block inverses) */
+  int32_t graph;          /* stab_solve_leading's first pass: 0 issued kernel by kernel, 1 captured
+                             into a HIP graph and launched, 2 replayed from the graph of an earlier
+                             call with the same batch layout (STAB_GRAPH=0 turns graphs off) */
+  int32_t lu_launches;    /* LU launches (one per run of points sharing a work-group shape) */
 } stab_stats;
 
 /* Context on HIP device `device`.  New in this library (the reference keeps

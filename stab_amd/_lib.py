@@ -46,7 +46,8 @@ class Stats(C.Structure):
                 ("ms_eig", C.c_double), ("ms_total", C.c_double), ("lu_flops", C.c_int64),
                 ("assemble_bytes", C.c_int64), ("npts", C.c_int32), ("nconv", C.c_int32),
                 ("restarts_max", C.c_int32), ("nfallback", C.c_int32), ("ms_arnoldi", C.c_double),
-                ("op_applies", C.c_int64), ("solve_bytes", C.c_double), ("ms_lu_kernel", C.c_double)]
+                ("op_applies", C.c_int64), ("solve_bytes", C.c_double), ("ms_lu_kernel", C.c_double),
+                ("graph", C.c_int32), ("lu_launches", C.c_int32)]
 
 
 _lib = None

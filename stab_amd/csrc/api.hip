@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <dlfcn.h>
 #include <map>
 #include <mutex>
 #include <string>
@@ -43,14 +44,26 @@ struct Arena {
   std::map<std::string, DevBuf> slots;
 };
 
+// A captured first pass of the leading-mode path and the launch parameters
+// it was captured with; replayed while a call has the same batch layout.
+struct GraphCache {
+  std::vector<long long> sig;
+  hipGraphExec_t exec = nullptr;
+};
+
 struct stab_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t stream2 = nullptr;        // second stream of the segmented LU launch (default priority)
+  hipEvent_t fork = nullptr, join = nullptr;
   std::string err;
   std::map<int, Profile> profiles;
   std::map<std::string, DevBuf> bufs;
   hipEvent_t ev[8];
   Arena arena;
+  std::map<int, int> lu_nt;             // tuned work-group size of the LU per padded order
+  GraphCache graph;
+  int graphs_captured = 0, graphs_replayed = 0;
 };
 
 namespace {
@@ -71,6 +84,11 @@ int fail(stab_ctx *c, int code, const char *fmt, ...) {
     if (e_ != hipSuccess)                                                             \
       return fail(ctx, STAB_E_HIP, "%s failed: %s", #x, hipGetErrorString(e_));       \
   } while (0)
+
+int env_int(const char *name, int dflt) {
+  const char *v = getenv(name);
+  return v ? atoi(v) : dflt;
+}
 
 constexpr size_t ARENA_ALIGN = 256;
 inline size_t arena_round(size_t b) { return (std::max(b, (size_t)ARENA_ALIGN) + ARENA_ALIGN - 1) / ARENA_ALIGN * ARENA_ALIGN; }
@@ -348,34 +366,162 @@ int run_prep(stab_ctx *c, int npts, Batch &B, bool equilibrate) {
   return STAB_OK;
 }
 
+// Work-group size of the LU for padded order npad: STAB_LU_NT when set (the
+// autotuner's override), else the tuning table, else 256 threads (two
+// work-groups per CU) up to order 768 and 512 above.  At most three panel rows
+// per thread.
+inline int lu_key(size_t elem, int npad) { return (elem == sizeof(zf) ? 0 : 1 << 20) + npad; }
+
+int lu_threads(const stab_ctx *c, size_t elem, int npad) {
+  const int forced = env_int("STAB_LU_NT", 0);
+  int nt = 0;
+  if (forced == 256 || forced == 512) nt = forced;
+  if (!nt) {
+    auto it = c->lu_nt.find(lu_key(elem, npad));
+    if (it != c->lu_nt.end()) nt = it->second;
+  }
+  if (!nt) nt = (npad <= 768) ? 256 : 512;
+  if (npad > 3 * nt) nt = 512;
+  return nt;
+}
+
+// The tuning table written by stab_amd/autotune.py: lines "f32|f64 npad
+// threads" ('#' comments), from STAB_LU_TUNING or data/lu_tuning.txt beside
+// this library.
+void load_lu_table(stab_ctx *c) {
+  std::string path;
+  if (const char *e = getenv("STAB_LU_TUNING")) {
+    path = e;
+  } else {
+    Dl_info info;
+    if (dladdr((void *)&load_lu_table, &info) && info.dli_fname) {
+      path = info.dli_fname;
+      const size_t k = path.rfind('/');
+      path = (k == std::string::npos ? std::string(".") : path.substr(0, k)) + "/data/lu_tuning.txt";
+    }
+  }
+  FILE *f = path.empty() ? nullptr : fopen(path.c_str(), "r");
+  if (!f) return;
+  char line[256];
+  while (fgets(line, sizeof line, f)) {
+    if (line[0] == '#') continue;
+    char prec[8] = {0};
+    int npad = 0, nt = 0;
+    if (sscanf(line, "%7s %d %d", prec, &npad, &nt) == 3 && npad > 0 && (nt == 256 || nt == 512) &&
+        npad <= 3 * nt && (!strcmp(prec, "f32") || !strcmp(prec, "f64")))
+      c->lu_nt[lu_key(!strcmp(prec, "f32") ? sizeof(zf) : sizeof(zd), npad)] = nt;
+  }
+  fclose(f);
+}
+
 template <typename T, int NB, int NT>
-int launch_lu_nt(stab_ctx *c, int npts, Batch &B, int left) {
-  const int R = (B.npadmax + NT - 1) / NT;
+int launch_lu_seg(stab_ctx *c, hipStream_t s, int R, int cnt, const int *order, Batch &B, int left) {
   cx<T> *mats = (cx<T> *)B.mats, *blks = (cx<T> *)B.blks;
   switch (R) {
-    case 1: hipLaunchKernelGGL((k_lu<T, NB, NT, 1, 16>), dim3(npts), dim3(NT), 0, c->stream, B.pts, mats, B.pivs, blks, B.info, nullptr, B.order, left); break;
-    case 2: hipLaunchKernelGGL((k_lu<T, NB, NT, 2, 16>), dim3(npts), dim3(NT), 0, c->stream, B.pts, mats, B.pivs, blks, B.info, nullptr, B.order, left); break;
-    case 3: hipLaunchKernelGGL((k_lu<T, NB, NT, 3, 16>), dim3(npts), dim3(NT), 0, c->stream, B.pts, mats, B.pivs, blks, B.info, nullptr, B.order, left); break;
-    default: return fail(c, STAB_E_ARG, "matrix order %d too large for the LU kernel", B.npadmax);
+    case 1: hipLaunchKernelGGL((k_lu<T, NB, NT, 1, 16>), dim3(cnt), dim3(NT), 0, s, B.pts, mats, B.pivs, blks, B.info, nullptr, order, left); break;
+    case 2: hipLaunchKernelGGL((k_lu<T, NB, NT, 2, 16>), dim3(cnt), dim3(NT), 0, s, B.pts, mats, B.pivs, blks, B.info, nullptr, order, left); break;
+    case 3: hipLaunchKernelGGL((k_lu<T, NB, NT, 3, 16>), dim3(cnt), dim3(NT), 0, s, B.pts, mats, B.pivs, blks, B.info, nullptr, order, left); break;
+    default: return fail(c, STAB_E_ARG, "matrix order too large for the LU kernel (%d panel rows)", R);
   }
   HIPCHECK(c, hipGetLastError());
   return STAB_OK;
 }
 
+// LU segments of one batch: runs of the largest-first order with the same
+// (work-group size, panel rows), so no point pays for the register budget of
+// a wider matrix.
+struct LuSeg { int nt, R, start, cnt; };
+std::vector<LuSeg> lu_segments(const stab_ctx *c, const Batch &B, size_t elem) {
+  std::vector<LuSeg> segs;
+  if (!env_int("STAB_LU_SEGMENTS", 1)) {     // one launch shaped by the widest matrix
+    const int nt = lu_threads(c, elem, B.npadmax);
+    segs.push_back({nt, (B.npadmax + nt - 1) / nt, 0, (int)B.horder.size()});
+    return segs;
+  }
+  for (int k = 0; k < (int)B.horder.size(); ++k) {
+    const int npad = B.h[B.horder[k]].npad, nt = lu_threads(c, elem, npad), R = (npad + nt - 1) / nt;
+    if (segs.empty() || segs.back().nt != nt || segs.back().R != R) segs.push_back({nt, R, k, 0});
+    ++segs.back().cnt;
+  }
+  return segs;
+}
+
 // lapack_form: L in final row order (stab_lu_factors); otherwise the
 // interchanges of each block are left out of the columns before it and the
-// solves apply them block by block.
+// solves apply them block by block.  Several segments run on two streams
+// (the first on the context stream), joined before the block inverses.
 template <typename T, int NB>
-int launch_lu(stab_ctx *c, int npts, Batch &B, bool lapack_form = false) {
-  // 4 waves (two work-groups per CU) up to order 768, 8 waves up to 1280;
-  // at most three panel rows per thread
+int launch_lu(stab_ctx *c, int npts, Batch &B, bool lapack_form = false, bool timed = true) {
   const int left = lapack_form ? 1 : 0;
-  int rc = (B.npadmax <= 768) ? launch_lu_nt<T, NB, 256>(c, npts, B, left) : launch_lu_nt<T, NB, 512>(c, npts, B, left);
-  if (rc != STAB_OK) return rc;
-  HIPCHECK(c, hipEventRecord(c->ev[7], c->stream));   // end of k_lu (its start is ev[2] in the solve paths)
+  const std::vector<LuSeg> segs = lu_segments(c, B, sizeof(cx<T>));
+  const bool two = segs.size() > 1 && env_int("STAB_LU_STREAMS", 2) >= 2;
+  if (two) {
+    HIPCHECK(c, hipEventRecord(c->fork, c->stream));
+    HIPCHECK(c, hipStreamWaitEvent(c->stream2, c->fork, 0));
+  }
+  for (size_t k = 0; k < segs.size(); ++k) {
+    const LuSeg &g = segs[k];
+    hipStream_t s = (two && (k & 1)) ? c->stream2 : c->stream;
+    const int *ord = B.order + g.start;
+    const int rc = (g.nt == 256) ? launch_lu_seg<T, NB, 256>(c, s, g.R, g.cnt, ord, B, left)
+                                 : launch_lu_seg<T, NB, 512>(c, s, g.R, g.cnt, ord, B, left);
+    if (rc != STAB_OK) return rc;
+  }
+  if (two) {
+    HIPCHECK(c, hipEventRecord(c->join, c->stream2));
+    HIPCHECK(c, hipStreamWaitEvent(c->stream, c->join, 0));
+  }
+  (void)npts;
+  if (timed) HIPCHECK(c, hipEventRecord(c->ev[7], c->stream));   // end of k_lu (its start is ev[2] in the solve paths)
   hipLaunchKernelGGL((k_blkinv<T, SOLVE_BS>), dim3((B.npadmax + SOLVE_BS - 1) / SOLVE_BS, npts), dim3(4 * SOLVE_BS), 0,
                      c->stream, B.pts, (const cx<T> *)B.mats, (cx<T> *)B.blks);
   HIPCHECK(c, hipGetLastError());
+  return STAB_OK;
+}
+
+// Replay the context's captured graph when `sig` matches the one it was
+// captured with; otherwise capture `issue` on the context stream (the work it
+// enqueues, forks to the second stream included), instantiate and launch it.
+// A capture the runtime refuses falls back to issuing the work directly.
+template <typename F>
+int run_graph(stab_ctx *c, const std::vector<long long> &sig, F issue, int *how) {
+  GraphCache &g = c->graph;
+  *how = 0;
+  if (g.exec && g.sig == sig) {
+    HIPCHECK(c, hipGraphLaunch(g.exec, c->stream));
+    ++c->graphs_replayed;
+    *how = 2;
+    return STAB_OK;
+  }
+  if (g.exec) {
+    (void)hipGraphExecDestroy(g.exec);
+    g.exec = nullptr;
+    g.sig.clear();
+  }
+  if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    (void)hipGetLastError();
+    return issue();
+  }
+  const int rc = issue();
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+  if (rc != STAB_OK || e != hipSuccess || !graph) {
+    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipGetLastError();
+    return issue();
+  }
+  hipGraphExec_t exec = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ei != hipSuccess || !exec) {
+    (void)hipGetLastError();
+    return issue();
+  }
+  g.exec = exec;
+  g.sig = sig;
+  ++c->graphs_captured;
+  HIPCHECK(c, hipGraphLaunch(g.exec, c->stream));
+  *how = 1;
   return STAB_OK;
 }
 
@@ -437,14 +583,13 @@ int run_eig(stab_ctx *c, int npts, Batch &B, const KSParams &kp, int &cycles_max
 
 double elapsed(hipEvent_t a, hipEvent_t b) {
   float ms = 0.f;
-  (void)hipEventElapsedTime(&ms, a, b);
+  if (hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+    (void)hipGetLastError();       // a failed timing query must not surface as a later call's error
+    return 0.0;
+  }
   return (double)ms;
 }
 
-int env_int(const char *name, int dflt) {
-  const char *v = getenv(name);
-  return v ? atoi(v) : dflt;
-}
 
 // Krylov-Schur basis size for a batch whose largest pair count is nevmax:
 // 24 (keep 12) up to 4 pairs, 40 (keep 20) above.
@@ -631,6 +776,13 @@ int stab_ctx_create(stab_ctx **out, int device) {
     return STAB_E_HIP;
   }
   for (auto &e : c->ev) (void)hipEventCreate(&e);
+  if (hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->join, hipEventDisableTiming) != hipSuccess) {
+    delete c;
+    return STAB_E_HIP;
+  }
+  load_lu_table(c);
   if (scratch_init(c) != STAB_OK) {
     for (auto &e : c->ev) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->stream);
@@ -648,6 +800,11 @@ int stab_ctx_destroy(stab_ctx *c) {
   for (auto &kv : c->bufs)
     if (kv.second.p) (void)hipFree(kv.second.p);
   for (auto &e : c->ev) (void)hipEventDestroy(e);
+  if (c->graph.exec) (void)hipGraphExecDestroy(c->graph.exec);
+  (void)hipStreamSynchronize(c->stream2);
+  (void)hipEventDestroy(c->fork);
+  (void)hipEventDestroy(c->join);
+  (void)hipStreamDestroy(c->stream2);
   (void)hipStreamDestroy(c->stream);
   delete c;
   return STAB_OK;
@@ -1128,34 +1285,70 @@ static int lead_invit(stab_ctx *c, int npts, const stab_point *pts, int mode, in
   if ((rc = build_batch(c, npts, pts, f32 ? sizeof(zf) : sizeof(zd), 1, B)) != STAB_OK) return rc;
   const size_t lds = op_lds_bytes(B.npadmax, B.nymax, 1, B.npadmax);
   if (lds > 150 * 1024) return fail(c, STAB_E_ARG, "inverse-iteration workspace %zu B exceeds LDS", lds);
-  HIPCHECK(c, hipEventRecord(c->ev[0], c->stream));
-  if ((rc = run_prep(c, npts, B, f32)) != STAB_OK) return rc;
-  HIPCHECK(c, hipEventRecord(c->ev[1], c->stream));
-  const int rb = (B.npadmax + 255) / 256;
-  if (f32)
-    hipLaunchKernelGGL((k_assemble_rows<float>), dim3(rb, npts), dim3(256), 0, c->stream, B.pts, B.coef, B.ops,
-                       B.scales, (zf *)B.mats, 1);
-  else
-    hipLaunchKernelGGL((k_assemble_rows<double>), dim3(rb, npts), dim3(256), 0, c->stream, B.pts, B.coef, B.ops,
-                       B.scales, (zd *)B.mats, 0);
-  HIPCHECK(c, hipGetLastError());
-  HIPCHECK(c, hipEventRecord(c->ev[2], c->stream));
-  if ((rc = f32 ? launch_lu<float, 16>(c, npts, B) : launch_lu<double, 8>(c, npts, B)) != STAB_OK) return rc;
-  HIPCHECK(c, hipEventRecord(c->ev[3], c->stream));
   // iterate to a relative residual of 1e-15 (or the rounding floor): with
   // kappa up to ~1e6 (these norms) that leaves <= 1e-9 relative eigenvalue
   // error, and 1e-16 costs about one more solve per point
   const double tol = 1e-15;
-  if (f32)
-    hipLaunchKernelGGL((k_invit<float, 16, 256>), dim3(npts), dim3(256), lds, c->stream, B.pts, (const zf *)B.mats,
-                       B.pivs, (const zf *)B.blks, B.scales, B.coef, B.ops, B.norms, maxit, tol, B.eval, B.evec,
-                       B.resid, B.ff, B.iters, B.flag, B.npadmax, B.nymax, B.order);
-  else
-    hipLaunchKernelGGL((k_invit<double, 16, 256>), dim3(npts), dim3(256), lds, c->stream, B.pts, (const zd *)B.mats,
-                       B.pivs, (const zd *)B.blks, B.scales, B.coef, B.ops, B.norms, maxit, tol, B.eval, B.evec,
-                       B.resid, B.ff, B.iters, B.flag, B.npadmax, B.nymax, B.order);
-  HIPCHECK(c, hipGetLastError());
-  HIPCHECK(c, hipEventRecord(c->ev[4], c->stream));
+  // The first pass (operators, node coefficients, assembly, LU, block
+  // inverses, inverse iteration) depends on the host only through the point
+  // descriptors, which build_batch has copied; it is issued as one unit and
+  // replayed from a captured graph while the batch layout stays the same.
+  int graph_how = 0;
+  const bool use_graph = env_int("STAB_GRAPH", 1) && !c->arena.on;
+  // phase events only when the kernels are issued directly: a graph is timed
+  // as a whole by events recorded around its launch
+  auto mark = [&](int k) -> int {
+    if (!use_graph) HIPCHECK(c, hipEventRecord(c->ev[k], c->stream));
+    return STAB_OK;
+  };
+  auto first_pass = [&]() -> int {
+    int rc2;
+    if ((rc2 = mark(0)) != STAB_OK) return rc2;
+    if ((rc2 = run_prep(c, npts, B, f32)) != STAB_OK) return rc2;
+    if ((rc2 = mark(1)) != STAB_OK) return rc2;
+    const int rb = (B.npadmax + 255) / 256;
+    if (f32)
+      hipLaunchKernelGGL((k_assemble_rows<float>), dim3(rb, npts), dim3(256), 0, c->stream, B.pts, B.coef, B.ops,
+                         B.scales, (zf *)B.mats, 1);
+    else
+      hipLaunchKernelGGL((k_assemble_rows<double>), dim3(rb, npts), dim3(256), 0, c->stream, B.pts, B.coef, B.ops,
+                         B.scales, (zd *)B.mats, 0);
+    HIPCHECK(c, hipGetLastError());
+    if ((rc2 = mark(2)) != STAB_OK) return rc2;
+    if ((rc2 = f32 ? launch_lu<float, 16>(c, npts, B, false, !use_graph) : launch_lu<double, 8>(c, npts, B, false, !use_graph)) != STAB_OK)
+      return rc2;
+    if ((rc2 = mark(3)) != STAB_OK) return rc2;
+    if (f32)
+      hipLaunchKernelGGL((k_invit<float, 16, 256>), dim3(npts), dim3(256), lds, c->stream, B.pts, (const zf *)B.mats,
+                         B.pivs, (const zf *)B.blks, B.scales, B.coef, B.ops, B.norms, maxit, tol, B.eval, B.evec,
+                         B.resid, B.ff, B.iters, B.flag, B.npadmax, B.nymax, B.order);
+    else
+      hipLaunchKernelGGL((k_invit<double, 16, 256>), dim3(npts), dim3(256), lds, c->stream, B.pts, (const zd *)B.mats,
+                         B.pivs, (const zd *)B.blks, B.scales, B.coef, B.ops, B.norms, maxit, tol, B.eval, B.evec,
+                         B.resid, B.ff, B.iters, B.flag, B.npadmax, B.nymax, B.order);
+    HIPCHECK(c, hipGetLastError());
+    return mark(4);
+  };
+  if (use_graph) {
+    std::vector<long long> sig = {npts, f32, maxit, (long long)lds, B.npadmax, B.nymax, B.nvecmax,
+                                  (long long)B.nscales};
+    for (int ny : B.nys) { sig.push_back(ny); sig.push_back(B.op_off[ny]); }
+    for (const void *q : {(const void *)B.pts, (const void *)B.prof, (const void *)B.ops, (const void *)B.coef,
+                          (const void *)B.grid, (const void *)B.mats, (const void *)B.pivs, (const void *)B.blks,
+                          (const void *)B.scales, (const void *)B.norms, (const void *)B.eval, (const void *)B.evec,
+                          (const void *)B.resid, (const void *)B.ff, (const void *)B.iters, (const void *)B.flag,
+                          (const void *)B.info, (const void *)B.order})
+      sig.push_back((long long)(uintptr_t)q);
+    for (const LuSeg &g : lu_segments(c, B, f32 ? sizeof(zf) : sizeof(zd))) {
+      sig.push_back(g.nt); sig.push_back(g.R); sig.push_back(g.cnt);
+    }
+    HIPCHECK(c, hipEventRecord(c->ev[0], c->stream));
+    rc = run_graph(c, sig, first_pass, &graph_how);
+    if (rc == STAB_OK) HIPCHECK(c, hipEventRecord(c->ev[4], c->stream));
+  } else {
+    rc = first_pass();
+  }
+  if (rc != STAB_OK) return rc;
   std::vector<zd> ev(npts);
   std::vector<double> rs(npts), ff(npts);
   std::vector<int> it(npts), fl(npts), inf(npts);
@@ -1254,16 +1447,22 @@ static int lead_invit(stab_ctx *c, int npts, const stab_point *pts, int mode, in
   }
   if (stats) {
     RunInfo ri;
-    ri.ms_prep = elapsed(c->ev[0], c->ev[1]);
-    ri.ms_assemble = elapsed(c->ev[1], c->ev[2]);
-    ri.ms_lu = elapsed(c->ev[2], c->ev[3]);
-    ri.ms_lu_kernel = elapsed(c->ev[2], c->ev[7]);
-    ri.ms_eig = elapsed(c->ev[3], c->ev[4]) + ms2;
+    if (!use_graph) {     // phase times exist only for direct launches
+      ri.ms_prep = elapsed(c->ev[0], c->ev[1]);
+      ri.ms_assemble = elapsed(c->ev[1], c->ev[2]);
+      ri.ms_lu = elapsed(c->ev[2], c->ev[3]);
+      ri.ms_lu_kernel = elapsed(c->ev[2], c->ev[7]);
+      ri.ms_eig = elapsed(c->ev[3], c->ev[4]) + ms2;
+    } else {
+      ri.ms_eig = ms2;
+    }
     ri.ms_total = elapsed(c->ev[0], c->ev[4]) + ms2;
     ri.ms_arnoldi = ri.ms_eig;
     ri.cycles = 1;
     ri.nconv = found;
     fill_stats(stats, npts, pts, B, ri, f32, it);
+    stats->graph = graph_how;
+    stats->lu_launches = (int32_t)lu_segments(c, B, f32 ? sizeof(zf) : sizeof(zd)).size();
   }
   return STAB_OK;
 }
