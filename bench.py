@@ -52,7 +52,9 @@ sys.path.insert(0, ROOT)
 PEAK_HBM_GBS = 8000.0
 PEAK_FP32_MFMA_TFLOPS = 157.3
 PEAK_FP64_MFMA_TFLOPS = 78.6
-LU32_KERNEL = "k_lu<float, 16, 256, 3, 16>"
+# the FP32 LU: per 64-column block step, one panel launch per work-group shape
+# and one trailing-update launch
+LU32_KERNELS = ("k_lu_panel<float, ", "k_lu_trail<float>")
 
 
 def make_workload(npts: int, seed: int):
@@ -83,27 +85,32 @@ def features(al, be, re):
     return np.stack([al, be, np.log(re), np.full(n, 0.5), np.full(n, 1.0)], 1).astype(np.float32)
 
 
-def load_traffic(kernel: str, key: str):
-    """HBM bytes per launch of `kernel` from the committed counter summary
-    whose workload-key line equals `key` (profiles/*_hbm_traffic.md), or None."""
+def load_traffic(key: str):
+    """HBM bytes per step of each kernel, {name: bytes}, from the committed
+    counter summary whose workload-key line equals `key`
+    (profiles/*_hbm_traffic.md, last column in GB per step), or {}."""
     import glob
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*_hbm_traffic.md"))):
         text = open(path).read()
         if f"workload-key: {key}\n" not in text:
             continue
+        rows = {}
         for line in text.splitlines():
             cells = [c.strip() for c in line.strip().strip("|").split("|")]
-            if len(cells) == 5 and cells[0] == kernel:
-                return float(cells[4]) * 1e9
-    return None
+            if len(cells) == 5 and cells[0].startswith("k_"):
+                try:
+                    rows[cells[0]] = float(cells[4]) * 1e9
+                except ValueError:
+                    pass
+        return rows
+    return {}
 
 
-def lu_traffic(launches: int, key: str):
-    """HBM bytes of the LU phase: the sum over its launches (one per work-group
-    shape) from the committed counter summary, or None."""
-    names = [LU32_KERNEL] if launches == 1 else ["k_lu<float, 16, 256, 3, 16>", "k_lu<float, 16, 256, 2, 16>"]
-    vals = [load_traffic(k, key) for k in names]
-    return None if any(v is None for v in vals) else float(sum(vals))
+def lu_traffic(key: str):
+    """HBM bytes of the FP32 LU per step: the sum over its panel and trailing
+    launches from the committed counter summary, or None."""
+    vals = [v for k, v in load_traffic(key).items() if k.startswith(LU32_KERNELS)]
+    return float(sum(vals)) if vals else None
 
 
 def load_qz_fixture(al, seed):
@@ -451,15 +458,14 @@ def main():
         })
         lu_ms = chosen["ms_lu_kernel"]
         achieved = chosen["lu_flops"] / (lu_ms * 1e-3) / 1e12
-        lu_kernel = (LU32_KERNEL + " (batched complex LU, FP32 matrix cores, one launch)" if chosen["lu_launches"] == 1
-                     else "k_lu<float, 16, 256, 3, 16> (N = 128 points) + k_lu<float, 16, 256, 2, 16> (N <= 96 points), "
-                          "concurrent on two streams; achieved = LU flops / span of both launches")
+        lu_kernel = ("FP32 batched complex LU: per 64-column block step k_lu_panel<float, 16, 256, R, 16> (R = 3 for "
+                     "N = 128, 2 for N <= 96) and k_lu_trail<float> (matrix-core update), "
+                     f"{chosen['lu_launches']} segments on concurrent streams; achieved = LU flops / span of the LU")
         out["roofline"] = {
             "bound": "mfma", "kernel": lu_kernel,
             "achieved": achieved, "peak": PEAK_FP32_MFMA_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_MFMA_TFLOPS,
-            "traffic": lu_traffic(chosen["lu_launches"], f"temporal-sweep points={args.points} seed={args.seed} "
-                                                                 f"path=chosen"),
+            "traffic": lu_traffic(f"temporal-sweep points={args.points} seed={args.seed} path=chosen"),
             "algorithmic_flops": "8/3 n^3 real flops per n x n complex LU (n = 5N), summed over the batch",
             "measured_copy_gbs": copy_gbs,
         }

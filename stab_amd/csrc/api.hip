@@ -6,6 +6,7 @@
 #include "lu.hip"
 #include "eig.hip"
 #include "mlp.hip"
+#include "lu_plan.h"
 
 #include <algorithm>
 #include <cmath>
@@ -51,11 +52,15 @@ struct GraphCache {
   hipGraphExec_t exec = nullptr;
 };
 
+constexpr int LU_STREAMS_MAX = 4;
+
 struct stab_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
-  hipStream_t stream2 = nullptr;        // second stream of the segmented LU launch (default priority)
-  hipEvent_t fork = nullptr, join = nullptr;
+  // further streams of the segmented LU launch (default priority): segment k
+  // runs on lus[k] (lus[0] is `stream`), forked from and joined to `stream`
+  hipStream_t lus[LU_STREAMS_MAX] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t fork = nullptr, join[LU_STREAMS_MAX] = {nullptr, nullptr, nullptr, nullptr};
   std::string err;
   std::map<int, Profile> profiles;
   std::map<std::string, DevBuf> bufs;
@@ -175,6 +180,47 @@ int check_point(stab_ctx *c, const stab_point &p, int idx) {
 constexpr int LU_ALIGN = 32;
 inline int round_up(int a, int b) { return (a + b - 1) / b * b; }
 
+// Work-group size of the LU for padded order npad: STAB_LU_NT when set (the
+// autotuner's override), else the tuning table, else 256 threads (two
+// work-groups per CU) up to order 768 and 512 above.  At most three panel rows
+// per thread.
+inline int lu_key(size_t elem, int npad) { return (elem == sizeof(zf) ? 0 : 1 << 20) + npad; }
+
+int lu_threads(const stab_ctx *c, size_t elem, int npad) {
+  const int forced = env_int("STAB_LU_NT", 0);
+  int nt = 0;
+  if (forced == 256 || forced == 512) nt = forced;
+  if (!nt) {
+    auto it = c->lu_nt.find(lu_key(elem, npad));
+    if (it != c->lu_nt.end()) nt = it->second;
+  }
+  if (!nt) nt = (npad <= 768) ? 256 : 512;
+  if (npad > 3 * nt) nt = 512;
+  return nt;
+}
+
+// Streams of the segmented LU: STAB_LU_STREAMS (1 .. LU_STREAMS_MAX), default 2.
+inline int lu_streams() { return std::max(1, std::min(LU_STREAMS_MAX, env_int("STAB_LU_STREAMS", 2))); }
+
+// LU segments of one batch (lu_plan.h): runs of the largest-first order with
+// the same (work-group size, panel rows), so no point pays for the register
+// budget of a wider matrix.
+std::vector<LuSeg> lu_segments(const stab_ctx *c, const std::vector<PointDev> &h, const std::vector<int> &horder,
+                               int npadmax, size_t elem) {
+  std::vector<LuSeg> segs;
+  if (!env_int("STAB_LU_SEGMENTS", 1)) {     // one segment shaped by the widest matrix
+    const int nt = lu_threads(c, elem, npadmax);
+    segs.push_back({nt, (npadmax + nt - 1) / nt, 0, (int)horder.size(), {}});
+    return segs;
+  }
+  for (int k = 0; k < (int)horder.size(); ++k) {
+    const int npad = h[horder[k]].npad, nt = lu_threads(c, elem, npad), R = (npad + nt - 1) / nt;
+    if (segs.empty() || segs.back().nt != nt || segs.back().R != R) segs.push_back({nt, R, k, 0, {}});
+    ++segs.back().cnt;
+  }
+  return segs;
+}
+
 // Layout of one batch on the device.
 struct Batch {
   std::vector<PointDev> h;
@@ -188,7 +234,11 @@ struct Batch {
   long long *doff = nullptr;
   zd *lead_vec = nullptr;
   int *pivs = nullptr, *info = nullptr;
-  int *order = nullptr;   // point indexes by decreasing npad (work-group -> point in k_lu, k_invit)
+  int *order = nullptr;   // point indexes by decreasing npad (work-group -> point in the LU, k_invit)
+  int *tasks = nullptr;   // trailing tasks of the LU steps (lu_plan)
+  std::vector<int> htask;
+  std::vector<LuSeg> segs;
+  size_t seg_elem = 0;
   KSState *state = nullptr;
   int npadmax = 0, nymax = 0, nvecmax = 0;
   size_t nscales = 0;
@@ -199,6 +249,7 @@ struct Batch {
 // Device buffers of one batch, in the order build_batch takes them.
 struct BatchSizes {
   long long poff = 0, ooff = 0, coff = 0, moff = 0, voff = 0, pvoff = 0, boff = 0, soff = 0, eoff = 0, nvtot = 0;
+  long long ntask = 0;
   size_t elem = 0;
   int npts = 0;
   // (name, bytes) of every buffer
@@ -227,6 +278,7 @@ struct BatchSizes {
             {"flag", sizeof(int) * n},
             {"doff", sizeof(long long) * (n + 1)},
             {"order", sizeof(int) * n},
+            {"tasks", sizeof(int) * 2 * (ntask > 0 ? ntask : 1)},
             {"lead_vec", sizeof(zd) * (nvtot > 0 ? nvtot : 1)}};
   }
   size_t arena_bytes() const {
@@ -303,6 +355,20 @@ void layout_batch(stab_ctx *c, int npts, const stab_point *pts, size_t elem, int
   S.soff = soff; S.eoff = eoff; S.elem = elem; S.npts = npts;
   S.nvtot = 0;
   for (int i = 0; i < npts; ++i) S.nvtot += B.h[i].nvec;
+  // Largest matrices first: work-groups start roughly in index order, so the
+  // batch then ends on its cheapest points instead of on a few large ones.
+  B.horder.resize(npts);
+  for (int i = 0; i < npts; ++i) B.horder[i] = i;
+  std::stable_sort(B.horder.begin(), B.horder.end(), [&](int a, int b) { return B.h[a].npad > B.h[b].npad; });
+  B.segs = lu_segments(c, B.h, B.horder, B.npadmax, elem);
+  {
+    std::vector<int> npad(npts);
+    for (int i = 0; i < npts; ++i) npad[i] = B.h[i].npad;
+    if (env_int("STAB_LU_SEGMENTS", 1)) lu_split(B.segs, npad, B.horder, lu_streams());
+    lu_plan(B.segs, npad, B.horder, elem == sizeof(zf) ? lu_gw<float>() : lu_gw<double>(), B.htask);
+  }
+  B.seg_elem = elem;
+  S.ntask = (long long)(B.htask.size() / 2);
 }
 
 // Build the device layout for points [0, npts).  elem = sizeof LU element.
@@ -324,17 +390,15 @@ int build_batch(stab_ctx *c, int npts, const stab_point *pts, size_t elem, int m
     B.small = (zd *)ptr[k++]; B.state = (KSState *)ptr[k++]; B.info = (int *)ptr[k++]; B.eval = (zd *)ptr[k++];
     B.resid = (double *)ptr[k++]; B.evec = (zd *)ptr[k++]; B.ff = (double *)ptr[k++]; B.iters = (int *)ptr[k++];
     B.sel = (int *)ptr[k++]; B.flag = (int *)ptr[k++]; B.doff = (long long *)ptr[k++]; B.order = (int *)ptr[k++];
-    B.lead_vec = (zd *)ptr[k++];
+    B.tasks = (int *)ptr[k++]; B.lead_vec = (zd *)ptr[k++];
   }
   std::map<int, long long> prof_off;
   for (int i = 0; i < npts; ++i) prof_off[pts[i].profile] = B.h[i].prof_off;
   HIPCHECK(c, hipMemcpyAsync(B.pts, B.h.data(), sizeof(PointDev) * npts, hipMemcpyHostToDevice, c->stream));
-  // Largest matrices first: work-groups start roughly in index order, so the
-  // batch then ends on its cheapest points instead of on a few large ones.
-  B.horder.resize(npts);
-  for (int i = 0; i < npts; ++i) B.horder[i] = i;
-  std::stable_sort(B.horder.begin(), B.horder.end(), [&](int a, int b) { return B.h[a].npad > B.h[b].npad; });
   HIPCHECK(c, hipMemcpyAsync(B.order, B.horder.data(), sizeof(int) * npts, hipMemcpyHostToDevice, c->stream));
+  if (!B.htask.empty())
+    HIPCHECK(c, hipMemcpyAsync(B.tasks, B.htask.data(), sizeof(int) * B.htask.size(), hipMemcpyHostToDevice,
+                               c->stream));
   for (auto &kv : prof_off) {
     const Profile &pr = c->profiles[kv.first];
     HIPCHECK(c, hipMemcpyAsync(B.prof + kv.second, pr.data.data(), sizeof(double) * pr.data.size(),
@@ -366,25 +430,6 @@ int run_prep(stab_ctx *c, int npts, Batch &B, bool equilibrate) {
   return STAB_OK;
 }
 
-// Work-group size of the LU for padded order npad: STAB_LU_NT when set (the
-// autotuner's override), else the tuning table, else 256 threads (two
-// work-groups per CU) up to order 768 and 512 above.  At most three panel rows
-// per thread.
-inline int lu_key(size_t elem, int npad) { return (elem == sizeof(zf) ? 0 : 1 << 20) + npad; }
-
-int lu_threads(const stab_ctx *c, size_t elem, int npad) {
-  const int forced = env_int("STAB_LU_NT", 0);
-  int nt = 0;
-  if (forced == 256 || forced == 512) nt = forced;
-  if (!nt) {
-    auto it = c->lu_nt.find(lu_key(elem, npad));
-    if (it != c->lu_nt.end()) nt = it->second;
-  }
-  if (!nt) nt = (npad <= 768) ? 256 : 512;
-  if (npad > 3 * nt) nt = 512;
-  return nt;
-}
-
 // The tuning table written by stab_amd/autotune.py: lines "f32|f64 npad
 // threads" ('#' comments), from STAB_LU_TUNING or data/lu_tuning.txt beside
 // this library.
@@ -414,65 +459,58 @@ void load_lu_table(stab_ctx *c) {
   fclose(f);
 }
 
+// One block step of one segment: the panel kernel, then the trailing kernel.
 template <typename T, int NB, int NT>
-int launch_lu_seg(stab_ctx *c, hipStream_t s, int R, int cnt, const int *order, Batch &B, int left) {
+int launch_lu_step(stab_ctx *c, hipStream_t s, int R, const LuStep &st, const int *order, Batch &B, int left) {
   cx<T> *mats = (cx<T> *)B.mats, *blks = (cx<T> *)B.blks;
-  switch (R) {
-    case 1: hipLaunchKernelGGL((k_lu<T, NB, NT, 1, 16>), dim3(cnt), dim3(NT), 0, s, B.pts, mats, B.pivs, blks, B.info, nullptr, order, left); break;
-    case 2: hipLaunchKernelGGL((k_lu<T, NB, NT, 2, 16>), dim3(cnt), dim3(NT), 0, s, B.pts, mats, B.pivs, blks, B.info, nullptr, order, left); break;
-    case 3: hipLaunchKernelGGL((k_lu<T, NB, NT, 3, 16>), dim3(cnt), dim3(NT), 0, s, B.pts, mats, B.pivs, blks, B.info, nullptr, order, left); break;
-    default: return fail(c, STAB_E_ARG, "matrix order too large for the LU kernel (%d panel rows)", R);
+  if (st.npanel > 0) {
+    switch (R) {
+      case 1: hipLaunchKernelGGL((k_lu_panel<T, NB, NT, 1, 16>), dim3(st.npanel), dim3(NT), 0, s, B.pts, mats, B.pivs, blks, B.info, order, left, st.K0); break;
+      case 2: hipLaunchKernelGGL((k_lu_panel<T, NB, NT, 2, 16>), dim3(st.npanel), dim3(NT), 0, s, B.pts, mats, B.pivs, blks, B.info, order, left, st.K0); break;
+      case 3: hipLaunchKernelGGL((k_lu_panel<T, NB, NT, 3, 16>), dim3(st.npanel), dim3(NT), 0, s, B.pts, mats, B.pivs, blks, B.info, order, left, st.K0); break;
+      default: return fail(c, STAB_E_ARG, "matrix order too large for the LU kernel (%d panel rows)", R);
+    }
+    HIPCHECK(c, hipGetLastError());
   }
-  HIPCHECK(c, hipGetLastError());
+  if (st.ntask > 0) {
+    hipLaunchKernelGGL((k_lu_trail<T>), dim3(st.ntask), dim3(256), 0, s, B.pts, mats, (const int *)B.pivs,
+                       (const cx<T> *)blks, (const int *)(B.tasks + 2 * (size_t)st.toff), st.K0);
+    HIPCHECK(c, hipGetLastError());
+  }
   return STAB_OK;
-}
-
-// LU segments of one batch: runs of the largest-first order with the same
-// (work-group size, panel rows), so no point pays for the register budget of
-// a wider matrix.
-struct LuSeg { int nt, R, start, cnt; };
-std::vector<LuSeg> lu_segments(const stab_ctx *c, const Batch &B, size_t elem) {
-  std::vector<LuSeg> segs;
-  if (!env_int("STAB_LU_SEGMENTS", 1)) {     // one launch shaped by the widest matrix
-    const int nt = lu_threads(c, elem, B.npadmax);
-    segs.push_back({nt, (B.npadmax + nt - 1) / nt, 0, (int)B.horder.size()});
-    return segs;
-  }
-  for (int k = 0; k < (int)B.horder.size(); ++k) {
-    const int npad = B.h[B.horder[k]].npad, nt = lu_threads(c, elem, npad), R = (npad + nt - 1) / nt;
-    if (segs.empty() || segs.back().nt != nt || segs.back().R != R) segs.push_back({nt, R, k, 0});
-    ++segs.back().cnt;
-  }
-  return segs;
 }
 
 // lapack_form: L in final row order (stab_lu_factors); otherwise the
 // interchanges of each block are left out of the columns before it and the
-// solves apply them block by block.  Several segments run on two streams
-// (the first on the context stream), joined before the block inverses.
+// solves apply them block by block.  Segment k runs on stream lus[k] (the
+// first on the context stream; at most LU_STREAMS_MAX segments), all joined
+// before the block inverses.
 template <typename T, int NB>
 int launch_lu(stab_ctx *c, int npts, Batch &B, bool lapack_form = false, bool timed = true) {
   const int left = lapack_form ? 1 : 0;
-  const std::vector<LuSeg> segs = lu_segments(c, B, sizeof(cx<T>));
-  const bool two = segs.size() > 1 && env_int("STAB_LU_STREAMS", 2) >= 2;
-  if (two) {
+  if (B.seg_elem != sizeof(cx<T>)) return fail(c, STAB_E_ARG, "LU plan built for another precision");
+  const std::vector<LuSeg> &segs = B.segs;
+  const int ns = std::min((int)segs.size(), lu_streams());
+  if (ns > 1) {
     HIPCHECK(c, hipEventRecord(c->fork, c->stream));
-    HIPCHECK(c, hipStreamWaitEvent(c->stream2, c->fork, 0));
+    for (int k = 1; k < ns; ++k) HIPCHECK(c, hipStreamWaitEvent(c->lus[k], c->fork, 0));
   }
   for (size_t k = 0; k < segs.size(); ++k) {
     const LuSeg &g = segs[k];
-    hipStream_t s = (two && (k & 1)) ? c->stream2 : c->stream;
+    hipStream_t s = c->lus[k % ns];
     const int *ord = B.order + g.start;
-    const int rc = (g.nt == 256) ? launch_lu_seg<T, NB, 256>(c, s, g.R, g.cnt, ord, B, left)
-                                 : launch_lu_seg<T, NB, 512>(c, s, g.R, g.cnt, ord, B, left);
-    if (rc != STAB_OK) return rc;
+    for (const LuStep &st : g.steps) {
+      const int rc = (g.nt == 256) ? launch_lu_step<T, NB, 256>(c, s, g.R, st, ord, B, left)
+                                   : launch_lu_step<T, NB, 512>(c, s, g.R, st, ord, B, left);
+      if (rc != STAB_OK) return rc;
+    }
   }
-  if (two) {
-    HIPCHECK(c, hipEventRecord(c->join, c->stream2));
-    HIPCHECK(c, hipStreamWaitEvent(c->stream, c->join, 0));
+  for (int k = 1; k < ns; ++k) {
+    HIPCHECK(c, hipEventRecord(c->join[k], c->lus[k]));
+    HIPCHECK(c, hipStreamWaitEvent(c->stream, c->join[k], 0));
   }
   (void)npts;
-  if (timed) HIPCHECK(c, hipEventRecord(c->ev[7], c->stream));   // end of k_lu (its start is ev[2] in the solve paths)
+  if (timed) HIPCHECK(c, hipEventRecord(c->ev[7], c->stream));   // end of the LU (its start is ev[2] in the solve paths)
   hipLaunchKernelGGL((k_blkinv<T, SOLVE_BS>), dim3((B.npadmax + SOLVE_BS - 1) / SOLVE_BS, npts), dim3(4 * SOLVE_BS), 0,
                      c->stream, B.pts, (const cx<T> *)B.mats, (cx<T> *)B.blks);
   HIPCHECK(c, hipGetLastError());
@@ -776,9 +814,12 @@ int stab_ctx_create(stab_ctx **out, int device) {
     return STAB_E_HIP;
   }
   for (auto &e : c->ev) (void)hipEventCreate(&e);
-  if (hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->join, hipEventDisableTiming) != hipSuccess) {
+  c->lus[0] = c->stream;
+  bool ok = hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) == hipSuccess;
+  for (int k = 1; k < LU_STREAMS_MAX && ok; ++k)
+    ok = hipStreamCreateWithFlags(&c->lus[k], hipStreamNonBlocking) == hipSuccess &&
+         hipEventCreateWithFlags(&c->join[k], hipEventDisableTiming) == hipSuccess;
+  if (!ok) {
     delete c;
     return STAB_E_HIP;
   }
@@ -801,10 +842,12 @@ int stab_ctx_destroy(stab_ctx *c) {
     if (kv.second.p) (void)hipFree(kv.second.p);
   for (auto &e : c->ev) (void)hipEventDestroy(e);
   if (c->graph.exec) (void)hipGraphExecDestroy(c->graph.exec);
-  (void)hipStreamSynchronize(c->stream2);
+  for (int k = 1; k < LU_STREAMS_MAX; ++k) {
+    if (c->lus[k]) (void)hipStreamSynchronize(c->lus[k]);
+    if (c->join[k]) (void)hipEventDestroy(c->join[k]);
+    if (c->lus[k]) (void)hipStreamDestroy(c->lus[k]);
+  }
   (void)hipEventDestroy(c->fork);
-  (void)hipEventDestroy(c->join);
-  (void)hipStreamDestroy(c->stream2);
   (void)hipStreamDestroy(c->stream);
   delete c;
   return STAB_OK;
@@ -1337,10 +1380,12 @@ static int lead_invit(stab_ctx *c, int npts, const stab_point *pts, int mode, in
                           (const void *)B.grid, (const void *)B.mats, (const void *)B.pivs, (const void *)B.blks,
                           (const void *)B.scales, (const void *)B.norms, (const void *)B.eval, (const void *)B.evec,
                           (const void *)B.resid, (const void *)B.ff, (const void *)B.iters, (const void *)B.flag,
-                          (const void *)B.info, (const void *)B.order})
+                          (const void *)B.info, (const void *)B.order, (const void *)B.tasks})
       sig.push_back((long long)(uintptr_t)q);
-    for (const LuSeg &g : lu_segments(c, B, f32 ? sizeof(zf) : sizeof(zd))) {
+    // the launch grids of the LU steps are part of the graph
+    for (const LuSeg &g : B.segs) {
       sig.push_back(g.nt); sig.push_back(g.R); sig.push_back(g.cnt);
+      for (const LuStep &st : g.steps) { sig.push_back(st.npanel); sig.push_back(st.ntask); sig.push_back(st.toff); }
     }
     HIPCHECK(c, hipEventRecord(c->ev[0], c->stream));
     rc = run_graph(c, sig, first_pass, &graph_how);
@@ -1462,7 +1507,7 @@ static int lead_invit(stab_ctx *c, int npts, const stab_point *pts, int mode, in
     ri.nconv = found;
     fill_stats(stats, npts, pts, B, ri, f32, it);
     stats->graph = graph_how;
-    stats->lu_launches = (int32_t)lu_segments(c, B, f32 ? sizeof(zf) : sizeof(zd)).size();
+    stats->lu_launches = (int32_t)B.segs.size();
   }
   return STAB_OK;
 }

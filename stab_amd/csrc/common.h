@@ -112,8 +112,19 @@ struct PointDev {
 //           (LINPACK's order of work).  Record b: ns, then ns touched rows
 //           srow[] at +1 and their source rows sorig[] at +1+128:
 //           y[srow[e]] <- y[sorig[e]].
+// The records are written in both modes.  After them, one trailing record per
+// block (LU_TREC ints), the block's net interchanges in the form the update of
+// the columns right of it reads them: nout, then topsrc[64] (input row of each
+// row of the block top), outrow[64] and outsrc[64] (rows below the block that
+// receive an input row of the block top).  Only the LU reads these.
 #define LU_PREC 257
-__host__ __device__ inline long long piv_elems(int npad) { return 2LL * npad + 1 + (long long)LU_PREC * ((npad + 63) / 64); }
+#define LU_TREC 193
+__host__ __device__ inline long long piv_elems(int npad) {
+  return 2LL * npad + 1 + (long long)(LU_PREC + LU_TREC) * ((npad + 63) / 64);
+}
+__host__ __device__ inline long long lu_trec_off(int npad, int block) {
+  return 2LL * npad + 1 + (long long)LU_PREC * ((npad + 63) / 64) + (long long)LU_TREC * block;
+}
 
 // Matrix forms produced by the assembly kernel
 enum {

@@ -75,34 +75,48 @@ __device__ inline int net_perm(const int *ipiv, int k0, int w, int *srow, int *s
   return ns;
 }
 
-// Two work-groups per CU: at most 256 registers per lane in all variants.
+// The batched LU in two launches per 64-column block (K0 = 0, 64, ...) over
+// the whole batch:
+//   k_lu_panel  one work-group per matrix whose order exceeds K0: factors the
+//               block column [K0, K0 + kb) (panels in registers, the in-block
+//               strips, the interchanges), stores the block's records, and
+//               forms X = inv(L11);
+//   k_lu_trail  one work-group per 64-column strip (32 in FP64) right of the
+//               block, for every such matrix: the block's row permutation,
+//               U12 = X A12 and A22 -= L21 U12 on the matrix cores.
+// Split this way, the strips of one matrix run on several CUs at once and the
+// latency-bound panel steps of some matrices overlap the matrix-core updates
+// of others; the arithmetic is that of one work-group doing both in turn.
 template <typename T, int NB, int NT, int R, int SW>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_lu(const PointDev *__restrict__ pts, cx<T> *__restrict__ mats,
-                                           int *__restrict__ pivs, cx<T> *__restrict__ blks,
-                                           int *__restrict__ info, const int *__restrict__ active,
-                                           const int *__restrict__ order, int left) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_lu_panel(const PointDev *__restrict__ pts, cx<T> *__restrict__ mats,
+                                                 int *__restrict__ pivs, cx<T> *__restrict__ blks,
+                                                 int *__restrict__ info, const int *__restrict__ order, int left,
+                                                 int K0) {
   typedef cx<T> C;
   // work-group -> point: `order` lists the points largest first, so the
   // batch's last work-groups are its cheapest
-  const int pi = order ? order[blockIdx.x] : blockIdx.x;
-  if (active && !active[pi]) return;
+  const int pi = order[blockIdx.x];
   const PointDev &P = pts[pi];
   const int n = P.npad, lda = P.npad;
+  if (K0 >= n) return;
   C *A = mats + P.mat_off;
   int *ipiv = pivs + P.piv_off;
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   constexpr int NW = NT / 64;
-  constexpr int KB = 64, GW = lu_gw<T>(), UP = GW + 1, NPB = KB / NB;
+  constexpr int KB = 64, SU = 2 * KB * 16;
   static_assert(KB % NB == 0, "panel width must divide the block width");
   static_assert(LU_PREC == 1 + 4 * KB, "interchange record: count, 2 KB rows, 2 KB sources");
+  static_assert(LU_TREC == 1 + 3 * KB, "trailing record: count, KB top sources, KB out rows, KB out sources");
 
   // double-buffered by column parity: a column's slots are rewritten two
-  // columns later, after a barrier every wave passes only once it has read them
-  __shared__ C s_prow2[2][NB], s_crow2[2][NB];
+  // columns later, after a barrier every wave passes only once it has read
+  // them.  s_cand2: each wave's best candidate row, s_crow2: the row of the
+  // diagonal; both are published before the column's one barrier.
+  __shared__ C s_cand2[2][NW][NB], s_crow2[2][NB];
   __shared__ T s_rv2[2][NW];
   __shared__ int s_ri2[2][NW];
   __shared__ C s_blk[NB * NB];
-  __shared__ C s_linv[NPB][NB * NB];      // inv(L11) of each panel of the block
+  __shared__ C s_linv[NB * NB];           // inv(L11) of the current panel
   __shared__ int s_piv[KB];
   __shared__ int s_srow[2 * KB], s_sorig[2 * KB];
   __shared__ int s_slot[1280];            // net_perm's row map (npad <= 1280)
@@ -110,18 +124,19 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
   __shared__ int s_nout, s_ns;
   __shared__ C s_stage[2 * NB * SW];
   __shared__ C s_u12[NB * SW];
-  __shared__ C s_u[KB * UP];
+  // packed strictly lower L11 for X (KB (KB - 1) / 2 entries), staging of the
+  // left-column interchanges (at most 2 KB rows x 16 columns), the final row list
+  __shared__ C s_u[SU];
 
   C Pn[R][NB];
-  int myinfo = 0;
+  int myinfo = (K0 == 0) ? 0 : info[pi];   // the first zero pivot of the earlier blocks
   for (int r = t; r < 1280; r += NT) s_slot[r] = -1;
   __syncthreads();
 
-  for (int K0 = 0; K0 < n; K0 += KB) {
+  {
     const int kb = min(KB, n - K0);       // 64, or 32 for the last block (n is a multiple of 32)
     const int kend = K0 + kb;
     for (int k0 = K0; k0 < kend; k0 += NB) {
-      const int pb = (k0 - K0) / NB;
       // ---- load the panel rows owned by this thread
 #pragma unroll
       for (int j = 0; j < R; ++j) {
@@ -139,7 +154,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
 #pragma unroll PANEL_UNROLL
       for (int c = 0; c < NB; ++c) {
         const int col = k0 + c;
-        C *s_prow = s_prow2[c & 1], *s_crow = s_crow2[c & 1];
+        C *s_crow = s_crow2[c & 1];
         T *s_rv = s_rv2[c & 1];
         int *s_ri = s_ri2[c & 1];
         T best = (T)-1;
@@ -161,21 +176,17 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
           const int oi = __shfl_xor(bi, off);
           if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
         }
+        // publish the wave's candidate (value, row index, the row itself) and
+        // the diagonal row; one barrier then serves the pivot choice and the
+        // interchange.  The wave's candidate rows are its own rows, so the
+        // owner of row bi is a lane of this wave.
         if (lane == 0) { s_rv[wv] = best; s_ri[wv] = bi; }
-        __syncthreads();
-        best = s_rv[0]; bi = s_ri[0];
-#pragma unroll
-        for (int w = 1; w < NW; ++w) {
-          if (s_rv[w] > best || (s_rv[w] == best && s_ri[w] < bi)) { best = s_rv[w]; bi = s_ri[w]; }
-        }
-        // no finite candidate (NaN column): keep the diagonal so every index stays in range
-        const int p = (bi >= col && bi < n) ? bi : col;
-        const int pj = p / NT, cj = col / NT;
+        const int cj = col / NT, bj = (bi < n) ? bi / NT : -1;
 #pragma unroll
         for (int j = 0; j < R; ++j) {
-          if (j == pj && (p % NT) == t) {
+          if (j == bj && (bi % NT) == t) {
 #pragma unroll
-            for (int cc = 0; cc < NB; ++cc) s_prow[cc] = Pn[j][cc];
+            for (int cc = 0; cc < NB; ++cc) s_cand2[c & 1][wv][cc] = Pn[j][cc];
           }
           if (j == cj && (col % NT) == t) {
 #pragma unroll
@@ -183,6 +194,17 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
           }
         }
         __syncthreads();
+        best = s_rv[0]; bi = s_ri[0];
+        int wb = 0;
+#pragma unroll
+        for (int w = 1; w < NW; ++w) {
+          if (s_rv[w] > best || (s_rv[w] == best && s_ri[w] < bi)) { best = s_rv[w]; bi = s_ri[w]; wb = w; }
+        }
+        // no finite candidate (NaN column): keep the diagonal so every index stays in range
+        const bool found = (bi >= col && bi < n);
+        const int p = found ? bi : col;
+        const C *s_prow = found ? s_cand2[c & 1][wb] : s_crow;
+        const int pj = p / NT;
         if (p != col) {
 #pragma unroll
           for (int j = 0; j < R; ++j) {
@@ -233,16 +255,20 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
         }
       }
       __syncthreads();
-      // ---- inv(L11) (unit lower), one column per thread
+      // ---- inv(L11) (unit lower), one column per thread, the column in
+      // registers (the terms k < c vanish: x[k] = 0 there)
       if (t < NB) {
         const int c = t;
-        C *Li = s_linv[pb];
+        C x[NB];
+#pragma unroll
         for (int i = 0; i < NB; ++i) {
           C s = (i == c) ? mk<T>(1, 0) : czero<T>();
-          if (i > c)
-            for (int k = c; k < i; ++k) s = cfms(s, s_blk[i * NB + k], Li[k * NB + c]);
-          Li[i * NB + c] = (i < c) ? czero<T>() : s;
+#pragma unroll
+          for (int k = 0; k < i; ++k) s = cfms(s, (k >= c) ? s_blk[i * NB + k] : czero<T>(), x[k]);
+          x[i] = (i < c) ? czero<T>() : s;
         }
+#pragma unroll
+        for (int i = 0; i < NB; ++i) s_linv[i * NB + c] = x[i];
       }
       __syncthreads();
       if (t < NB) ipiv[k0 + t] = s_piv[k0 - K0 + t];
@@ -263,7 +289,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
       __syncthreads();
       // ---- the rest of the block column, in strips of SW columns
       const int nout = s_nout, nslot = NB + nout;
-      const C *Li = s_linv[pb];
+      const C *Li = s_linv;
       for (int c0 = k0 + NB; c0 < kend; c0 += SW) {
         const int sw = min(SW, kend - c0);      // strips never cross the block end
         for (int e = t; e < nslot * sw; e += NT) {
@@ -351,15 +377,19 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
     }
     __syncthreads();
     const int bns = s_ns, bnout = s_nout;
-    if (!left) {
-      // ---- the block's net interchanges for the forward solves (pivot mode 1)
+    {
+      // ---- the block's net interchanges: the record of the forward solves
+      // (read in pivot mode 1) and the trailing record of the update kernel
       int *rec = ipiv + 2 * n + 1 + (size_t)(K0 / KB) * LU_PREC;
       for (int e = t; e < bns; e += NT) { rec[1 + e] = s_srow[e]; rec[1 + 2 * KB + e] = s_sorig[e]; }
       if (t == 0) rec[0] = bns;
+      int *trec = ipiv + lu_trec_off(n, K0 / KB);
+      for (int e = t; e < kb; e += NT) trec[1 + e] = s_topsrc[e];
+      for (int e = t; e < bnout; e += NT) { trec[1 + KB + e] = s_outrow[e]; trec[1 + 2 * KB + e] = s_outsrc[e]; }
+      if (t == 0) trec[0] = bnout;
     }
     // ---- left columns [0, K0) (pivot mode 0 only): move the touched rows,
-    // 16 columns at a time, through LDS (s_u is idle here; bns * 16 <= 2 KB * 16
-    // entries fit)
+    // 16 columns at a time, through LDS (bns * 16 <= 2 KB * 16 = SU entries)
     if (left && bns > 0 && K0 > 0) {
       for (int c0 = 0; c0 < K0; c0 += 16) {
         for (int e = t; e < bns * 16; e += NT) s_u[e] = A[(size_t)(c0 + e / bns) * lda + s_sorig[e % bns]];
@@ -374,10 +404,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
     // per column: lane q of column c keeps x[m] = X(4 m + q, c) in registers
     // and sums the terms k = q (mod 4) of each substitution step; the sums meet
     // by shuffles.  The strictly lower triangle of L11 is staged in s_u, packed
-    // by rows (row i at i (i - 1) / 2; 2016 entries, which s_u holds for both
-    // strip widths), with zeros in rows and columns past kb.
+    // by rows (row i at i (i - 1) / 2; 2016 entries), with zeros in rows and
+    // columns past kb.
     C *X = blks + P.blk_off + (size_t)(K0 / KB) * KB * KB;
-    static_assert(KB * (KB - 1) / 2 <= KB * UP, "packed L11 fits in s_u");
+    static_assert(KB * (KB - 1) / 2 <= SU, "packed L11 fits in s_u");
     for (int e = t; e < KB * KB; e += NT) {
       const int i = e % KB, k = e / KB;
       if (k < i) s_u[i * (i - 1) / 2 + k] = (i < kb) ? A[(size_t)(K0 + k) * lda + K0 + i] : czero<T>();
@@ -407,148 +437,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
       for (int m = 0; m < KB / 4; ++m) X[c * KB + 4 * m + q] = x[m];
     }
     __syncthreads();
-    // ---- trailing columns [kend, n), GW at a time: permute, U12 = X A12 and
-    // A22 -= L21 U12, both on the matrix cores
-    if (kend < n) {
-      typedef typename Mfma<T>::v4 v4;
-      constexpr int CT = GW / 16, RT = 2;
-      for (int c0 = kend; c0 < n; c0 += GW) {
-        const int gw = min(GW, n - c0);
-        // stage the block rows of the strip in final order (some come from
-        // below the block); then the rows below that change receive their
-        // input rows, which lie in the block top and are not written before
-        // U12 is stored
-        for (int e = t; e < kb * gw; e += NT) {
-          const int k = e % kb, j = e / kb;
-          s_u[k * UP + j] = A[(size_t)(c0 + j) * lda + s_topsrc[k]];
-        }
-        __syncthreads();
-        for (int e = t; e < bnout * gw; e += NT) {
-          const int o = e % bnout, j = e / bnout;
-          A[(size_t)(c0 + j) * lda + s_outrow[o]] = A[(size_t)(c0 + j) * lda + s_outsrc[o]];
-        }
-        // U12 = X S (S = the staged rows) on the matrix cores, transposed as
-        // the update below: wave w takes column tile w % CTN and the row tiles
-        // it owns (four per column tile, split over NW / CTN waves so that the
-        // triangular work, 4 (rt + 1) depth-4 steps for row tile rt, balances)
-        {
-          constexpr int CTN = GW / 16, NWC = NW / CTN;
-          static_assert(NWC == 1 || NWC == 2 || NWC == 4, "waves per U12 column tile");
-          // X is the same for every strip; an opaque copy of its address keeps
-          // the compiler from hoisting its loads out of the strip loop, where
-          // they would stay live across the update and spill
-          const C *Xs = X;
-          asm volatile("" : "+s"(Xs));
-          const int ct = wv % CTN, h = wv / CTN;
-          const bool colok = ct * 16 < gw;
-          v4 ur[4], ui[4];
-#pragma unroll
-          for (int rt = 0; rt < 4; ++rt) {
-#pragma unroll
-            for (int i = 0; i < 4; ++i) { ur[rt][i] = (T)0; ui[rt][i] = (T)0; }
-            const int own = (NWC == 1) ? 0 : (NWC == 2 ? ((rt == 0 || rt == 3) ? 0 : 1) : rt);
-            if (own != h || !colok) continue;
-            // X(i, k) = 0 for k > i: row tile rt needs k < 16 (rt + 1)
-#pragma unroll 4
-            for (int ks = 0; ks < 4 * (rt + 1); ++ks) {
-              const int k = 4 * ks + (lane >> 4);
-              const C sv = s_u[k * UP + ct * 16 + (lane & 15)];
-              const C xv = Xs[(size_t)k * KB + rt * 16 + (lane & 15)];
-              ur[rt] = Mfma<T>::mac(sv.x, xv.x, ur[rt]);
-              ur[rt] = Mfma<T>::mac(-sv.y, xv.y, ur[rt]);
-              ui[rt] = Mfma<T>::mac(sv.y, xv.x, ui[rt]);
-              ui[rt] = Mfma<T>::mac(sv.x, xv.y, ui[rt]);
-            }
-          }
-          __syncthreads();     // S fully read
-#pragma unroll
-          for (int rt = 0; rt < 4; ++rt) {
-            const int own = (NWC == 1) ? 0 : (NWC == 2 ? ((rt == 0 || rt == 3) ? 0 : 1) : rt);
-            if (own != h || !colok) continue;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-              const int j = ct * 16 + Mfma<T>::row_of(lane, i), row = rt * 16 + (lane & 15);
-              if (j < gw) {
-                const C u = mk<T>(ur[rt][i], ui[rt][i]);
-                s_u[row * UP + j] = u;
-                A[(size_t)(c0 + j) * lda + K0 + row] = u;
-              }
-            }
-          }
-          __syncthreads();
-        }
-        // A22 -= L21 U12: each wave takes chunks of 32 rows (two 16-row
-        // tiles) against the gw columns of the strip
-        for (int r0 = kend + 32 * wv; r0 < n; r0 += 32 * NW) {
-          v4 dr[CT][RT], di[CT][RT];
-#pragma unroll
-          for (int ct = 0; ct < CT; ++ct)
-#pragma unroll
-            for (int rt = 0; rt < RT; ++rt)
-#pragma unroll
-              for (int i = 0; i < 4; ++i) {
-                C x = czero<T>();
-                if (ct * 16 < gw)
-                  x = A[(size_t)(c0 + ct * 16 + Mfma<T>::row_of(lane, i)) * lda + r0 + rt * 16 + (lane & 15)];
-                dr[ct][rt][i] = x.x;
-                di[ct][rt][i] = x.y;
-              }
-          // L21 of the chunk is loaded LH depth-4 steps at a time before the
-          // MFMA chain that uses it (one memory latency per LH steps)
-          constexpr int LH = 8;
-          for (int kb0 = 0; kb0 < kb; kb0 += 4 * LH) {
-            T lr[LH][RT], li[LH][RT];
-#pragma unroll
-            for (int st = 0; st < LH; ++st) {
-              const int k = kb0 + 4 * st + (lane >> 4);
-#pragma unroll
-              for (int rt = 0; rt < RT; ++rt) {
-                C x = czero<T>();
-                if (kb0 + 4 * st < kb) x = A[(size_t)(K0 + k) * lda + r0 + rt * 16 + (lane & 15)];
-                lr[st][rt] = x.x;
-                li[st][rt] = x.y;
-              }
-            }
-#pragma unroll
-            for (int st = 0; st < LH; ++st) {
-              if (kb0 + 4 * st < kb) {
-                const int k = kb0 + 4 * st + (lane >> 4);
-#pragma unroll
-                for (int ct = 0; ct < CT; ++ct) {
-                  if (ct * 16 < gw) {
-                    const C uu = s_u[k * UP + ct * 16 + (lane & 15)];
-                    const T nur = -uu.x, ui = uu.y, nui = -uu.y;
-#pragma unroll
-                    for (int rt = 0; rt < RT; ++rt) {
-                      dr[ct][rt] = Mfma<T>::mac(nur, lr[st][rt], dr[ct][rt]);
-                      dr[ct][rt] = Mfma<T>::mac(ui, li[st][rt], dr[ct][rt]);
-                      di[ct][rt] = Mfma<T>::mac(nui, lr[st][rt], di[ct][rt]);
-                      di[ct][rt] = Mfma<T>::mac(nur, li[st][rt], di[ct][rt]);
-                    }
-                  }
-                }
-              }
-            }
-          }
-#pragma unroll
-          for (int ct = 0; ct < CT; ++ct)
-#pragma unroll
-            for (int rt = 0; rt < RT; ++rt)
-#pragma unroll
-              for (int i = 0; i < 4; ++i)
-                if (ct * 16 < gw)
-                  A[(size_t)(c0 + ct * 16 + Mfma<T>::row_of(lane, i)) * lda + r0 + rt * 16 + (lane & 15)] =
-                      mk<T>(dr[ct][rt][i], di[ct][rt][i]);
-        }
-        __syncthreads();
-      }
-    }
-  }  // blocks
   // ---- final row permutation: row r of L U is row perm[r] of the input,
   // composed in LDS (s_slot, idle now) by one thread, then stored; only the
   // LAPACK form (pivot mode 0) has it: the mode-1 solves use the block records
   static_assert(sizeof(s_u) >= 1280 * sizeof(int), "the pivots fit in s_u");
-  if (left) {
+  if (left && kend == n) {
     int *perm = ipiv + n;
     int *s_ip = reinterpret_cast<int *>(s_u);   // the pivots, read by one thread
     __syncthreads();
@@ -563,13 +456,176 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
     __syncthreads();
     for (int r = t; r < n; r += NT) perm[r] = s_slot[r];
   }
-  if (t == 0) { info[pi] = myinfo; ipiv[2 * n] = left ? 0 : 1; }
+  if (t == 0) {
+    info[pi] = myinfo;
+    if (kend == n) ipiv[2 * n] = left ? 0 : 1;
+  }
+  }  // block K0
+}
+
+
+// Strip [c0, c0 + gw) of the columns right of block K0 of one matrix
+// (tasks[2 b], tasks[2 b + 1] = point, c0; point -1 pads the grid): stage the
+// block rows of the strip in final order (some come from below the block;
+// the rows below that change then receive their input rows, which lie in
+// the block top and are not written before U12 is stored), U12 = X S with
+// X = inv(L11) of the block, and A22 -= L21 U12 over the rows below the
+// block.  Complex products as 2 x 2 real blocks on the matrix cores,
+// computed transposed (MFMA rows = matrix columns) so that each accumulator
+// register of 16 lanes covers 16 consecutive matrix rows.
+template <typename T>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_lu_trail(const PointDev *__restrict__ pts, cx<T> *__restrict__ mats,
+                                                  const int *__restrict__ pivs, const cx<T> *__restrict__ blks,
+                                                  const int *__restrict__ tasks, int K0) {
+  typedef cx<T> C;
+  typedef typename Mfma<T>::v4 v4;
+  constexpr int NT = 256, NW = NT / 64, KB = 64, GW = lu_gw<T>(), UP = GW + 1, CT = GW / 16, RT = 2;
+  const int pi = tasks[2 * blockIdx.x], c0 = tasks[2 * blockIdx.x + 1];
+  if (pi < 0) return;
+  const PointDev &P = pts[pi];
+  const int n = P.npad, lda = P.npad;
+  if (K0 >= n) return;
+  const int kb = min(KB, n - K0), kend = K0 + kb;
+  if (c0 < kend || c0 >= n) return;
+  C *A = mats + P.mat_off;
+  const int *trec = pivs + P.piv_off + lu_trec_off(n, K0 / KB);
+  const C *X = blks + P.blk_off + (size_t)(K0 / KB) * KB * KB;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  __shared__ int s_topsrc[KB], s_outrow[KB], s_outsrc[KB];
+  __shared__ C s_u[KB * UP];
+  const int nout = min(trec[0], KB);
+  for (int e = t; e < kb; e += NT) s_topsrc[e] = trec[1 + e];
+  for (int e = t; e < nout; e += NT) { s_outrow[e] = trec[1 + KB + e]; s_outsrc[e] = trec[1 + 2 * KB + e]; }
+  __syncthreads();
+  const int gw = min(GW, n - c0);
+  // stage the block rows of the strip in final order (some come from
+  // below the block); then the rows below that change receive their
+  // input rows, which lie in the block top and are not written before
+  // U12 is stored
+  for (int e = t; e < kb * gw; e += NT) {
+    const int k = e % kb, j = e / kb;
+    s_u[k * UP + j] = A[(size_t)(c0 + j) * lda + s_topsrc[k]];
+  }
+  __syncthreads();
+  for (int e = t; e < nout * gw; e += NT) {
+    const int o = e % nout, j = e / nout;
+    A[(size_t)(c0 + j) * lda + s_outrow[o]] = A[(size_t)(c0 + j) * lda + s_outsrc[o]];
+  }
+  // U12 = X S (S = the staged rows) on the matrix cores, transposed as
+  // the update below: wave w takes column tile w % CTN and the row tiles
+  // it owns (four per column tile, split over NW / CTN waves so that the
+  // triangular work, 4 (rt + 1) depth-4 steps for row tile rt, balances)
+  {
+    constexpr int CTN = GW / 16, NWC = NW / CTN;
+    static_assert(NWC == 1 || NWC == 2 || NWC == 4, "waves per U12 column tile");
+    const C *Xs = X;
+    const int ct = wv % CTN, h = wv / CTN;
+    const bool colok = ct * 16 < gw;
+    v4 ur[4], ui[4];
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) { ur[rt][i] = (T)0; ui[rt][i] = (T)0; }
+      const int own = (NWC == 1) ? 0 : (NWC == 2 ? ((rt == 0 || rt == 3) ? 0 : 1) : rt);
+      if (own != h || !colok) continue;
+      // X(i, k) = 0 for k > i: row tile rt needs k < 16 (rt + 1)
+#pragma unroll 4
+      for (int ks = 0; ks < 4 * (rt + 1); ++ks) {
+        const int k = 4 * ks + (lane >> 4);
+        const C sv = s_u[k * UP + ct * 16 + (lane & 15)];
+        const C xv = Xs[(size_t)k * KB + rt * 16 + (lane & 15)];
+        ur[rt] = Mfma<T>::mac(sv.x, xv.x, ur[rt]);
+        ur[rt] = Mfma<T>::mac(-sv.y, xv.y, ur[rt]);
+        ui[rt] = Mfma<T>::mac(sv.y, xv.x, ui[rt]);
+        ui[rt] = Mfma<T>::mac(sv.x, xv.y, ui[rt]);
+      }
+    }
+    __syncthreads();     // S fully read
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt) {
+      const int own = (NWC == 1) ? 0 : (NWC == 2 ? ((rt == 0 || rt == 3) ? 0 : 1) : rt);
+      if (own != h || !colok) continue;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int j = ct * 16 + Mfma<T>::row_of(lane, i), row = rt * 16 + (lane & 15);
+        if (j < gw) {
+          const C u = mk<T>(ur[rt][i], ui[rt][i]);
+          s_u[row * UP + j] = u;
+          A[(size_t)(c0 + j) * lda + K0 + row] = u;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // A22 -= L21 U12: each wave takes chunks of 32 rows (two 16-row
+  // tiles) against the gw columns of the strip
+  for (int r0 = kend + 32 * wv; r0 < n; r0 += 32 * NW) {
+    v4 dr[CT][RT], di[CT][RT];
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          C x = czero<T>();
+          if (ct * 16 < gw)
+            x = A[(size_t)(c0 + ct * 16 + Mfma<T>::row_of(lane, i)) * lda + r0 + rt * 16 + (lane & 15)];
+          dr[ct][rt][i] = x.x;
+          di[ct][rt][i] = x.y;
+        }
+    // L21 of the chunk is loaded LH depth-4 steps at a time before the
+    // MFMA chain that uses it (one memory latency per LH steps)
+    constexpr int LH = 8;
+    for (int kb0 = 0; kb0 < kb; kb0 += 4 * LH) {
+      T lr[LH][RT], li[LH][RT];
+#pragma unroll
+      for (int st = 0; st < LH; ++st) {
+        const int k = kb0 + 4 * st + (lane >> 4);
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt) {
+          C x = czero<T>();
+          if (kb0 + 4 * st < kb) x = A[(size_t)(K0 + k) * lda + r0 + rt * 16 + (lane & 15)];
+          lr[st][rt] = x.x;
+          li[st][rt] = x.y;
+        }
+      }
+#pragma unroll
+      for (int st = 0; st < LH; ++st) {
+        if (kb0 + 4 * st < kb) {
+          const int k = kb0 + 4 * st + (lane >> 4);
+#pragma unroll
+          for (int ct = 0; ct < CT; ++ct) {
+            if (ct * 16 < gw) {
+              const C uu = s_u[k * UP + ct * 16 + (lane & 15)];
+              const T nur = -uu.x, ui = uu.y, nui = -uu.y;
+#pragma unroll
+              for (int rt = 0; rt < RT; ++rt) {
+                dr[ct][rt] = Mfma<T>::mac(nur, lr[st][rt], dr[ct][rt]);
+                dr[ct][rt] = Mfma<T>::mac(ui, li[st][rt], dr[ct][rt]);
+                di[ct][rt] = Mfma<T>::mac(nui, lr[st][rt], di[ct][rt]);
+                di[ct][rt] = Mfma<T>::mac(nur, li[st][rt], di[ct][rt]);
+              }
+            }
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          if (ct * 16 < gw)
+            A[(size_t)(c0 + ct * 16 + Mfma<T>::row_of(lane, i)) * lda + r0 + rt * 16 + (lane & 15)] =
+                mk<T>(dr[ct][rt][i], di[ct][rt][i]);
+  }
 }
 
 // Inverses of the diagonal blocks of U (upper) of order BS = 64 (the last
 // block may be shorter; entries outside it are stored as zeros), in FP64
 // arithmetic, stored as T, column-major.  With nblk = ceil(npad / BS), block b
-// of inv(L) is at blk_off + b BS^2 (written by k_lu as it factors) and block b
+// of inv(L) is at blk_off + b BS^2 (written by k_lu_panel) and block b
 // of inv(U) at blk_off + (nblk + b) BS^2; the per-point allocation is
 // 2 nblk BS^2 elements (blk_elems above).
 // grid: x = block index, y = point; 256 threads, four lanes per column: lane
