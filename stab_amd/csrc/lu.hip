@@ -38,6 +38,7 @@ template <> struct Mfma<double> {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
   static __device__ inline int row_of(int lane, int i) { return 4 * i + (lane >> 4); }
+  static constexpr int row_step(int i) { return 4 * i; }   // row_of(lane, i) - row_of(lane, 0)
 };
 template <> struct Mfma<float> {
   typedef float v4 __attribute__((ext_vector_type(4)));
@@ -45,6 +46,7 @@ template <> struct Mfma<float> {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
   static __device__ inline int row_of(int lane, int i) { return 4 * (lane >> 4) + i; }
+  static constexpr int row_step(int i) { return i; }       // row_of(lane, i) - row_of(lane, 0)
 };
 
 // Columns of one LDS tile of U12 in the MFMA update (f64: 32, f32: 64).
@@ -143,7 +145,13 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
         const int r = t + j * NT;
 #pragma unroll
         for (int c = 0; c < NB; ++c)
-          Pn[j][c] = (r >= k0 && r < n) ? A[(size_t)(k0 + c) * lda + r] : czero<T>();
+        {
+          // unconditional load from a row in range, then select: a load under
+          // a divergent branch waits for its data before the branch joins
+          const bool ok = r >= k0 && r < n;
+          const C v = A[(size_t)(k0 + c) * lda + (ok ? r : k0)];
+          Pn[j][c] = ok ? v : czero<T>();
+        }
       }
       // ---- factor the panel.  Every register access below has compile-time
       // column indices (with predicates when the loop is not unrolled), so Pn
@@ -326,7 +334,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
             const bool live = (r >= k0 + NB && r < n);
             C acc[G];
 #pragma unroll
-            for (int q = 0; q < G; ++q) acc[q] = live ? A[(size_t)(c0 + cg + q) * lda + r] : czero<T>();
+            for (int q = 0; q < G; ++q) {
+              const C v = A[(size_t)(c0 + cg + q) * lda + (live ? r : k0)];
+              acc[q] = live ? v : czero<T>();
+            }
 #pragma unroll
             for (int k = 0; k < NB; ++k) {
 #pragma unroll
@@ -410,7 +421,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
     static_assert(KB * (KB - 1) / 2 <= SU, "packed L11 fits in s_u");
     for (int e = t; e < KB * KB; e += NT) {
       const int i = e % KB, k = e / KB;
-      if (k < i) s_u[i * (i - 1) / 2 + k] = (i < kb) ? A[(size_t)(K0 + k) * lda + K0 + i] : czero<T>();
+      if (k < i) {
+        const C v = A[(size_t)(K0 + k) * lda + K0 + (i < kb ? i : 0)];
+        s_u[i * (i - 1) / 2 + k] = (i < kb) ? v : czero<T>();
+      }
     }
     __syncthreads();
     if (t < 4 * KB) {
@@ -479,7 +493,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                                                   const int *__restrict__ tasks, int K0) {
   typedef cx<T> C;
   typedef typename Mfma<T>::v4 v4;
-  constexpr int NT = 256, NW = NT / 64, KB = 64, GW = lu_gw<T>(), UP = GW + 1, CT = GW / 16, RT = 2;
+  constexpr int NT = 256, NW = NT / 64, KB = 64, GW = lu_gw<T>(), UP = GW + 1, CT = GW / 16;
   const int pi = tasks[2 * blockIdx.x], c0 = tasks[2 * blockIdx.x + 1];
   if (pi < 0) return;
   const PointDev &P = pts[pi];
@@ -490,7 +504,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   C *A = mats + P.mat_off;
   const int *trec = pivs + P.piv_off + lu_trec_off(n, K0 / KB);
   const C *X = blks + P.blk_off + (size_t)(K0 / KB) * KB * KB;
-  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  // the wave index as a scalar: the row offsets of the update below are then
+  // wave-uniform and the addresses a per-lane base plus a scalar offset
+  const int t = threadIdx.x, lane = t & 63, wv = __builtin_amdgcn_readfirstlane(t >> 6);
   __shared__ int s_topsrc[KB], s_outrow[KB], s_outsrc[KB];
   __shared__ C s_u[KB * UP];
   const int nout = min(trec[0], KB);
@@ -502,9 +518,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   // below the block); then the rows below that change receive their
   // input rows, which lie in the block top and are not written before
   // U12 is stored
-  for (int e = t; e < kb * gw; e += NT) {
-    const int k = e % kb, j = e / kb;
-    s_u[k * UP + j] = A[(size_t)(c0 + j) * lda + s_topsrc[k]];
+  {
+    // every load of the thread issued before the first store to LDS
+    constexpr int NE = (KB * GW + NT - 1) / NT;
+    C v[NE];
+#pragma unroll
+    for (int q = 0; q < NE; ++q) {
+      const int e = t + q * NT, k = e % KB, j = e / KB;
+      const bool ok = k < kb && j < gw;       // else a harmless in-range load, not stored
+      v[q] = A[(size_t)(ok ? c0 + j : c0) * lda + (ok ? s_topsrc[k] : K0)];
+    }
+#pragma unroll
+    for (int q = 0; q < NE; ++q) {
+      const int e = t + q * NT, k = e % KB, j = e / KB;
+      if (k < kb && j < gw) s_u[k * UP + j] = v[q];
+    }
   }
   __syncthreads();
   for (int e = t; e < nout * gw; e += NT) {
@@ -529,7 +557,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
       const int own = (NWC == 1) ? 0 : (NWC == 2 ? ((rt == 0 || rt == 3) ? 0 : 1) : rt);
       if (own != h || !colok) continue;
       // X(i, k) = 0 for k > i: row tile rt needs k < 16 (rt + 1)
-#pragma unroll 4
+#pragma unroll
       for (int ks = 0; ks < 4 * (rt + 1); ++ks) {
         const int k = 4 * ks + (lane >> 4);
         const C sv = s_u[k * UP + ct * 16 + (lane & 15)];
@@ -557,68 +585,78 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     }
     __syncthreads();
   }
-  // A22 -= L21 U12: each wave takes chunks of 32 rows (two 16-row
-  // tiles) against the gw columns of the strip
-  for (int r0 = kend + 32 * wv; r0 < n; r0 += 32 * NW) {
-    v4 dr[CT][RT], di[CT][RT];
+  // A22 -= L21 U12: each wave takes chunks of 16 rows against the gw columns
+  // of the strip (rows of all chunks are multiples of 16 below n, a multiple
+  // of 32); the A22 tile of the wave's next chunk is loaded while the current
+  // one is on the matrix cores
+  // Addresses as one per-lane base plus wave-uniform offsets: tile entry
+  // (ct, i) of rows r0 .. r0 + 15 is Ab[(ct 16 + row_step(i)) lda + r0], and
+  // L21 entry of depth step s is Lb[(4 s) lda + r0].  Loads take clamped
+  // in-range offsets and are selected afterwards, so none waits under a branch.
+  const C *Lb = A + (size_t)(K0 + (lane >> 4)) * lda + (lane & 15);
+  C *Ab = A + (size_t)(c0 + Mfma<T>::row_of(lane, 0)) * lda + (lane & 15);
+  v4 dr[CT], di[CT], nr[CT], ni[CT];
+  int r0 = kend + 16 * wv;
+#pragma unroll
+  for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const bool ok = r0 < n && ct * 16 < gw;
+      C x = Ab[(ok ? (ct * 16 + Mfma<T>::row_step(i)) * lda + r0 : kend)];
+      if (!ok) x = czero<T>();
+      dr[ct][i] = x.x;
+      di[ct][i] = x.y;
+    }
+  for (; r0 < n; r0 += 16 * NW) {
+    const int rn = r0 + 16 * NW;
 #pragma unroll
     for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
-      for (int rt = 0; rt < RT; ++rt)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          C x = czero<T>();
-          if (ct * 16 < gw)
-            x = A[(size_t)(c0 + ct * 16 + Mfma<T>::row_of(lane, i)) * lda + r0 + rt * 16 + (lane & 15)];
-          dr[ct][rt][i] = x.x;
-          di[ct][rt][i] = x.y;
-        }
+      for (int i = 0; i < 4; ++i) {
+        const bool ok = rn < n && ct * 16 < gw;
+        C x = Ab[(ok ? (ct * 16 + Mfma<T>::row_step(i)) * lda + rn : r0)];
+        if (!ok) x = czero<T>();
+        nr[ct][i] = x.x;
+        ni[ct][i] = x.y;
+      }
     // L21 of the chunk is loaded LH depth-4 steps at a time before the
-    // MFMA chain that uses it (one memory latency per LH steps)
+    // MFMA chain that uses it (one memory latency per LH steps).  kb is 32 or
+    // 64, so every step of a 4 LH = 32 deep pass is inside the block; the
+    // column tiles past gw (the last strip of an order 32 mod 64) are
+    // computed from in-range data and not stored: a guard here would split
+    // the MFMA chain into branches that copy its accumulators
     constexpr int LH = 8;
+    static_assert(4 * LH == 32, "kb is a multiple of 32");
+#pragma unroll 1
     for (int kb0 = 0; kb0 < kb; kb0 += 4 * LH) {
-      T lr[LH][RT], li[LH][RT];
+      T lr[LH], li[LH];
+#pragma unroll
+      for (int st = 0; st < LH; ++st) {
+        const C x = Lb[(kb0 + 4 * st) * lda + r0];
+        lr[st] = x.x;
+        li[st] = x.y;
+      }
 #pragma unroll
       for (int st = 0; st < LH; ++st) {
         const int k = kb0 + 4 * st + (lane >> 4);
 #pragma unroll
-        for (int rt = 0; rt < RT; ++rt) {
-          C x = czero<T>();
-          if (kb0 + 4 * st < kb) x = A[(size_t)(K0 + k) * lda + r0 + rt * 16 + (lane & 15)];
-          lr[st][rt] = x.x;
-          li[st][rt] = x.y;
-        }
-      }
-#pragma unroll
-      for (int st = 0; st < LH; ++st) {
-        if (kb0 + 4 * st < kb) {
-          const int k = kb0 + 4 * st + (lane >> 4);
-#pragma unroll
-          for (int ct = 0; ct < CT; ++ct) {
-            if (ct * 16 < gw) {
-              const C uu = s_u[k * UP + ct * 16 + (lane & 15)];
-              const T nur = -uu.x, ui = uu.y, nui = -uu.y;
-#pragma unroll
-              for (int rt = 0; rt < RT; ++rt) {
-                dr[ct][rt] = Mfma<T>::mac(nur, lr[st][rt], dr[ct][rt]);
-                dr[ct][rt] = Mfma<T>::mac(ui, li[st][rt], dr[ct][rt]);
-                di[ct][rt] = Mfma<T>::mac(nui, lr[st][rt], di[ct][rt]);
-                di[ct][rt] = Mfma<T>::mac(nur, li[st][rt], di[ct][rt]);
-              }
-            }
-          }
+        for (int ct = 0; ct < CT; ++ct) {
+          const C uu = s_u[k * UP + ct * 16 + (lane & 15)];
+          const T nur = -uu.x, ui = uu.y, nui = -uu.y;
+          dr[ct] = Mfma<T>::mac(nur, lr[st], dr[ct]);
+          dr[ct] = Mfma<T>::mac(ui, li[st], dr[ct]);
+          di[ct] = Mfma<T>::mac(nui, lr[st], di[ct]);
+          di[ct] = Mfma<T>::mac(nur, li[st], di[ct]);
         }
       }
     }
 #pragma unroll
     for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
-      for (int rt = 0; rt < RT; ++rt)
+      for (int i = 0; i < 4; ++i)
+        if (ct * 16 < gw) Ab[(ct * 16 + Mfma<T>::row_step(i)) * lda + r0] = mk<T>(dr[ct][i], di[ct][i]);
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
-          if (ct * 16 < gw)
-            A[(size_t)(c0 + ct * 16 + Mfma<T>::row_of(lane, i)) * lda + r0 + rt * 16 + (lane & 15)] =
-                mk<T>(dr[ct][rt][i], di[ct][rt][i]);
+    for (int ct = 0; ct < CT; ++ct) { dr[ct] = nr[ct]; di[ct] = ni[ct]; }
   }
 }
 
@@ -649,7 +687,9 @@ __global__ __launch_bounds__(4 * BS) void k_blkinv(const PointDev *__restrict__ 
   __shared__ cx<T> s_a[BS * LD];  // row-major copy of the block, as stored (FP32: 33 KB, so more groups per CU)
   for (int e = t; e < BS * BS; e += 4 * BS) {
     const int i = e % BS, j = e / BS;   // consecutive lanes read consecutive rows of column j
-    s_a[i * LD + j] = (i < bs && j < bs) ? A[(size_t)(k0 + j) * n + k0 + i] : czero<T>();
+    const bool ok = i < bs && j < bs;
+    const cx<T> v = A[(size_t)(k0 + (ok ? j : 0)) * n + k0 + (ok ? i : 0)];
+    s_a[i * LD + j] = ok ? v : czero<T>();
   }
   __syncthreads();
   const int c = t >> 2, q = t & 3;
