@@ -63,6 +63,56 @@ template <typename T> __host__ __device__ inline bool scale_ok(cx<T> a) {
   return m > (T)0 && (a.x - a.x) == (T)0 && (a.y - a.y) == (T)0;   // x - x is NaN for Inf and NaN
 }
 
+// Lane moves by DPP: one VALU operation each, no trip through the LDS unit
+// that __shfl_xor's lane permute takes.  update_dpp(old, src, ctrl, row_mask,
+// bank_mask, bound_ctrl): a lane whose source lane is outside its row, or
+// whose row is masked off, keeps `old` (here its own value).
+template <int CTRL, int RM = 0xF> __device__ inline int dpp_move(int x) {
+  return __builtin_amdgcn_update_dpp(x, x, CTRL, RM, 0xF, false);
+}
+template <int CTRL, int RM = 0xF> __device__ inline float dpp_move(float x) {
+  return __int_as_float(dpp_move<CTRL, RM>(__float_as_int(x)));
+}
+template <int CTRL, int RM = 0xF> __device__ inline double dpp_move(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = dpp_move<CTRL, RM>((int)(unsigned int)(b & 0xffffffffLL));
+  const int hi = dpp_move<CTRL, RM>((int)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned int)hi << 32) | (unsigned int)lo));
+}
+// the value of lane 63 of the wave in every lane
+__device__ inline int lane63(int x) { return __builtin_amdgcn_readlane(x, 63); }
+__device__ inline float lane63(float x) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 63)); }
+__device__ inline double lane63(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)(unsigned int)(b & 0xffffffffLL), 63);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned int)hi << 32) | (unsigned int)lo));
+}
+// exchange with lane ^ 1 and lane ^ 2 (quad permutations [1 0 3 2], [2 3 0 1])
+constexpr int DPP_XOR1 = 0xB1, DPP_XOR2 = 0x4E;
+template <typename T> __device__ inline T quad_xor1(T x) { return dpp_move<DPP_XOR1>(x); }
+template <typename T> __device__ inline T quad_xor2(T x) { return dpp_move<DPP_XOR2>(x); }
+
+// Arg-max of (v, i) over the 64 lanes of a wave: larger v, and the lower i
+// among equal v, a total order when no v is NaN, so the result does not
+// depend on the order of the reduction; returned in every lane.  Row shifts
+// by 1, 2, 4 and 8 leave each 16-lane row's maximum in its lane 15, the row
+// broadcasts (15 to rows 1 and 3, 31 to rows 2 and 3) combine the rows into
+// lane 63, which is then read in every lane.
+template <typename T> __device__ inline void wave_argmax(T &v, int &i) {
+#define STAB_AM_STEP(CTRL, RM)                                                 \
+  {                                                                            \
+    const T ov = dpp_move<CTRL, RM>(v);                                        \
+    const int oi = dpp_move<CTRL, RM>(i);                                      \
+    if (ov > v || (ov == v && oi < i)) { v = ov; i = oi; }                     \
+  }
+  STAB_AM_STEP(0x111, 0xF) STAB_AM_STEP(0x112, 0xF) STAB_AM_STEP(0x114, 0xF) STAB_AM_STEP(0x118, 0xF)
+  STAB_AM_STEP(0x142, 0xA) STAB_AM_STEP(0x143, 0xC)
+#undef STAB_AM_STEP
+  v = lane63(v);
+  i = lane63(i);
+}
+
 template <typename To, typename Ti> __host__ __device__ inline cx<To> cvt(cx<Ti> a) { return mk<To>((To)a.x, (To)a.y); }
 
 // Per-node coefficient slots (each a 5x5 complex block, row-major p*5+q):

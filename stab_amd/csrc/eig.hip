@@ -143,8 +143,8 @@ __device__ inline void solve_update(const cx<T> *__restrict__ A, int lda, int k0
     for (int r = 0; r < NR; ++r)
 #pragma unroll
       for (int e = 0; e < E; ++e) {
-        a[r][e].x += __shfl_xor(a[r][e].x, 1); a[r][e].y += __shfl_xor(a[r][e].y, 1);
-        a[r][e].x += __shfl_xor(a[r][e].x, 2); a[r][e].y += __shfl_xor(a[r][e].y, 2);
+        a[r][e].x += quad_xor1(a[r][e].x); a[r][e].y += quad_xor1(a[r][e].y);
+        a[r][e].x += quad_xor2(a[r][e].x); a[r][e].y += quad_xor2(a[r][e].y);
       }
     if (row < r1 && q == 0) {
 #pragma unroll

@@ -178,12 +178,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
             if (v > best) { best = v; bi = r; }
           }
         }
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-          const T ov = __shfl_xor(best, off);
-          const int oi = __shfl_xor(bi, off);
-          if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
-        }
+        // best is -1 or a magnitude (a NaN one never replaces it), so the
+        // wave's arg-max is well defined
+        wave_argmax(best, bi);
         // publish the wave's candidate (value, row index, the row itself) and
         // the diagonal row; one barrier then serves the pivot choice and the
         // interchange.  The wave's candidate rows are its own rows, so the
@@ -441,8 +438,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
 #pragma unroll
           for (int m = 0; m <= mi; ++m)     // entry (i, 4 m + q) of L11 when 4 m + q < i
             s = cfma(s, (m < mi || q < r) ? s_u[i * (i - 1) / 2 + 4 * m + q] : czero<T>(), x[m]);
-          s.x += __shfl_xor(s.x, 1); s.y += __shfl_xor(s.y, 1);
-          s.x += __shfl_xor(s.x, 2); s.y += __shfl_xor(s.y, 2);
+          s.x += quad_xor1(s.x); s.y += quad_xor1(s.y);
+          s.x += quad_xor2(s.x); s.y += quad_xor2(s.y);
           const C v = (i < c || i >= kb || c >= kb) ? czero<T>() : ((i == c) ? mk<T>(1, 0) : czero<T>() - s);
           if (r == q) x[mi] = v;
         }
@@ -708,8 +705,8 @@ __global__ __launch_bounds__(4 * BS) void k_blkinv(const PointDev *__restrict__ 
         const int k = 4 * m + q;
         if (k > i && k <= c) s = cfma(s, cvt<double>(s_a[i * LD + k]), x[m]);
       }
-      s.x += __shfl_xor(s.x, 1); s.y += __shfl_xor(s.y, 1);
-      s.x += __shfl_xor(s.x, 2); s.y += __shfl_xor(s.y, 2);
+      s.x += quad_xor1(s.x); s.y += quad_xor1(s.y);
+      s.x += quad_xor2(s.x); s.y += quad_xor2(s.y);
       const zd rhs = ((i == c) ? mk(1.0, 0.0) : czero<double>()) - s;
       const zd v = (i > c || c >= bs) ? czero<double>() : cdiv(rhs, cvt<double>(s_a[i * LD + i]));
       if (r == q) x[mi] = v;
