@@ -126,7 +126,7 @@ typedef struct {
   int32_t graph;          /* stab_solve_leading's first pass: 0 issued kernel by kernel, 1 captured
                              into a HIP graph and launched, 2 replayed from the graph of an earlier
                              call with the same batch layout (STAB_GRAPH=0 turns graphs off) */
-  int32_t lu_launches;    /* LU launches (one per run of points sharing a work-group shape) */
+  int32_t lu_launches;    /* LU segments: runs of points sharing a work-group shape, split by work up to one per LU stream; each runs its 64-column block steps (a panel and a trailing launch each) on its own stream */
 } stab_stats;
 
 /* Context on HIP device `device`.  New in this library (the reference keeps

@@ -199,8 +199,8 @@ int lu_threads(const stab_ctx *c, size_t elem, int npad) {
   return nt;
 }
 
-// Streams of the segmented LU: STAB_LU_STREAMS (1 .. LU_STREAMS_MAX), default 2.
-inline int lu_streams() { return std::max(1, std::min(LU_STREAMS_MAX, env_int("STAB_LU_STREAMS", 2))); }
+// Streams of the segmented LU: STAB_LU_STREAMS (1 .. LU_STREAMS_MAX), default 4.
+inline int lu_streams() { return std::max(1, std::min(LU_STREAMS_MAX, env_int("STAB_LU_STREAMS", 4))); }
 
 // LU segments of one batch (lu_plan.h): runs of the largest-first order with
 // the same (work-group size, panel rows), so no point pays for the register

@@ -90,7 +90,7 @@ __device__ inline int net_perm(const int *ipiv, int k0, int w, int *srow, int *s
 // latency-bound panel steps of some matrices overlap the matrix-core updates
 // of others; the arithmetic is that of one work-group doing both in turn.
 template <typename T, int NB, int NT, int R, int SW>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_lu_panel(const PointDev *__restrict__ pts, cx<T> *__restrict__ mats,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(R <= 2 && NT == 256 ? 3 : 2))) void k_lu_panel(const PointDev *__restrict__ pts, cx<T> *__restrict__ mats,
                                                  int *__restrict__ pivs, cx<T> *__restrict__ blks,
                                                  int *__restrict__ info, const int *__restrict__ order, int left,
                                                  int K0) {

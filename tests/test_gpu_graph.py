@@ -104,3 +104,22 @@ def test_lu_segments_match_single_launch(ctx):
     assert ok.all()
     assert np.max(np.abs(one["evals"] - seg["evals"]) / np.abs(seg["evals"])) <= 1e-9
     assert np.all(seg["resid"] < 1e-13)
+
+
+def test_lu_streams_give_bitwise_equal_results(ctx):
+    """The LU's segments split further and spread over 1, 2 or 4 streams
+    (STAB_LU_STREAMS): every point's factorisation is the same arithmetic in
+    any grouping, so the leading modes, vectors, residuals and solve counts
+    are bitwise equal."""
+    pts = _points(60, 21)
+    runs = {}
+    try:
+        for ns in (1, 2, 4):
+            os.environ["STAB_LU_STREAMS"] = str(ns)
+            runs[ns] = _run(ctx, pts, False)
+            _say(f"lu streams {ns}: {runs[ns]['stats']['lu_launches']} segments")
+    finally:
+        os.environ.pop("STAB_LU_STREAMS", None)
+    assert runs[4]["stats"]["lu_launches"] >= runs[1]["stats"]["lu_launches"]
+    for ns in (2, 4):
+        _same(runs[ns], runs[1])
