@@ -1332,10 +1332,20 @@ static int lead_invit(stab_ctx *c, int npts, const stab_point *pts, int mode, in
   // kappa up to ~1e6 (these norms) that leaves <= 1e-9 relative eigenvalue
   // error, and 1e-16 costs about one more solve per point
   const double tol = 1e-15;
-  // The first pass (operators, node coefficients, assembly, LU, block
-  // inverses, inverse iteration) depends on the host only through the point
-  // descriptors, which build_batch has copied; it is issued as one unit and
-  // replayed from a captured graph while the batch layout stays the same.
+  // Second pass on the same factors (at most 2 maxit steps): two-vector
+  // iteration (k_invit2) for the temporal points the first pass missed, and
+  // for those it reached only after 3/4 of its steps (a second eigenvalue
+  // nearly as close to the shift, which may be the wanted one).  The
+  // work-groups choose their points on the device from the first pass's
+  // results, so both passes are issued together; the host applies the same
+  // rule afterwards to read the answers.
+  const size_t lds2 = lds + sizeof(zd) * (6 * (size_t)B.npadmax + 2 * 256);
+  const bool do2 = lds2 <= 160 * 1024;
+  // The whole pass (operators, node coefficients, assembly, LU, block
+  // inverses, inverse iteration, second pass) depends on the host only through
+  // the point descriptors, which build_batch has copied; it is issued as one
+  // unit and replayed from a captured graph while the batch layout stays the
+  // same.
   int graph_how = 0;
   const bool use_graph = env_int("STAB_GRAPH", 1) && !c->arena.on;
   // phase events only when the kernels are issued directly: a graph is timed
@@ -1370,11 +1380,27 @@ static int lead_invit(stab_ctx *c, int npts, const stab_point *pts, int mode, in
                          B.pivs, (const zd *)B.blks, B.scales, B.coef, B.ops, B.norms, maxit, tol, B.eval, B.evec,
                          B.resid, B.ff, B.iters, B.flag, B.npadmax, B.nymax, B.order);
     HIPCHECK(c, hipGetLastError());
-    return mark(4);
+    if ((rc2 = mark(4)) != STAB_OK) return rc2;
+    if (do2) {
+      zd *ev2 = B.eval + npts;
+      double *rs2 = B.resid + npts, *ff2 = B.ff + npts;    // 2 npts entries each (KS_NEVMAX npts allocated)
+      if (f32)
+        hipLaunchKernelGGL((k_invit2<float, 16, 256>), dim3(npts), dim3(256), lds2, c->stream, B.pts, nullptr,
+                           (const zf *)B.mats, B.pivs, (const zf *)B.blks, B.scales, B.coef, B.ops, B.norms, 2 * maxit,
+                           tol, nearest ? 1 : 0, ev2, B.evec, rs2, ff2, B.iters, B.npadmax, B.nymax, B.order, B.info,
+                           B.flag, B.eval, B.ff, maxit);
+      else
+        hipLaunchKernelGGL((k_invit2<double, 16, 256>), dim3(npts), dim3(256), lds2, c->stream, B.pts, nullptr,
+                           (const zd *)B.mats, B.pivs, (const zd *)B.blks, B.scales, B.coef, B.ops, B.norms, 2 * maxit,
+                           tol, nearest ? 1 : 0, ev2, B.evec, rs2, ff2, B.iters, B.npadmax, B.nymax, B.order, B.info,
+                           B.flag, B.eval, B.ff, maxit);
+      HIPCHECK(c, hipGetLastError());
+    }
+    return mark(6);
   };
   if (use_graph) {
     std::vector<long long> sig = {npts, f32, maxit, (long long)lds, B.npadmax, B.nymax, B.nvecmax,
-                                  (long long)B.nscales};
+                                  (long long)B.nscales, (long long)lds2, do2, nearest};
     for (int ny : B.nys) { sig.push_back(ny); sig.push_back(B.op_off[ny]); }
     for (const void *q : {(const void *)B.pts, (const void *)B.prof, (const void *)B.ops, (const void *)B.coef,
                           (const void *)B.grid, (const void *)B.mats, (const void *)B.pivs, (const void *)B.blks,
@@ -1389,7 +1415,7 @@ static int lead_invit(stab_ctx *c, int npts, const stab_point *pts, int mode, in
     }
     HIPCHECK(c, hipEventRecord(c->ev[0], c->stream));
     rc = run_graph(c, sig, first_pass, &graph_how);
-    if (rc == STAB_OK) HIPCHECK(c, hipEventRecord(c->ev[4], c->stream));
+    if (rc == STAB_OK) HIPCHECK(c, hipEventRecord(c->ev[6], c->stream));
   } else {
     rc = first_pass();
   }
@@ -1403,6 +1429,13 @@ static int lead_invit(stab_ctx *c, int npts, const stab_point *pts, int mode, in
   HIPCHECK(c, hipMemcpyAsync(it.data(), B.iters, sizeof(int) * npts, hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(c, hipMemcpyAsync(fl.data(), B.flag, sizeof(int) * npts, hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(c, hipMemcpyAsync(inf.data(), B.info, sizeof(int) * npts, hipMemcpyDeviceToHost, c->stream));
+  std::vector<zd> e2(2 * (size_t)npts);
+  std::vector<double> r2(2 * (size_t)npts), f2(2 * (size_t)npts);
+  if (do2) {
+    HIPCHECK(c, hipMemcpyAsync(e2.data(), B.eval + npts, sizeof(zd) * 2 * npts, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(r2.data(), B.resid + npts, sizeof(double) * 2 * npts, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(f2.data(), B.ff + npts, sizeof(double) * 2 * npts, hipMemcpyDeviceToHost, c->stream));
+  }
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   ok.assign(npts, 0);
   int found = 0;
@@ -1416,44 +1449,16 @@ static int lead_invit(stab_ctx *c, int npts, const stab_point *pts, int mode, in
     o.info[i] = inf[i] ? STAB_INFO_SINGULAR : (ok[i] ? STAB_INFO_OK : STAB_INFO_NOCONV);
     found += ok[i];
   }
-  // Second pass on the same factors (at most 2 maxit steps): two-vector
-  // iteration (k_invit2) for the
-  // temporal points the first pass missed, and for those it reached only
-  // after 3/4 of its steps (a second eigenvalue nearly as close to the shift,
-  // which may be the wanted one).  Of the two Ritz pairs the wanted one
+  // The points the second pass took (the rule k_invit2 applied on the
+  // device; `it` now holds both passes' solves, which only grow, so the rule
+  // still selects the same points).  Of the two Ritz pairs the wanted one
   // (leading discrete, or nearest the shift) with residual below 1e-14 is
   // taken, in column sel[i] of the point's vector block, unless the other
   // pair, not converged, could still outrank it (lead2_competes).
   std::vector<int> sel(npts, 0), sub;
   for (int i = 0; i < npts; ++i)
     if (!inf[i] && pts[i].itype == 1 && (!ok[i] || 4 * it[i] >= 3 * (maxit + 1))) sub.push_back(i);
-  std::stable_sort(sub.begin(), sub.end(), [&](int a, int b) { return B.h[a].npad > B.h[b].npad; });
-  const size_t lds2 = lds + sizeof(zd) * (6 * (size_t)B.npadmax + 2 * 256);
-  float ms2 = 0.f;
-  if (!sub.empty() && lds2 <= 160 * 1024) {
-    const int ns = (int)sub.size();
-    zd *ev2 = B.eval + npts;
-    double *rs2 = B.resid + npts, *ff2 = B.ff + npts;    // 2 npts entries each (KS_NEVMAX npts allocated)
-    HIPCHECK(c, hipMemcpyAsync(B.sel, sub.data(), sizeof(int) * ns, hipMemcpyHostToDevice, c->stream));
-    HIPCHECK(c, hipEventRecord(c->ev[5], c->stream));
-    if (f32)
-      hipLaunchKernelGGL((k_invit2<float, 16, 256>), dim3(ns), dim3(256), lds2, c->stream, B.pts, B.sel,
-                         (const zf *)B.mats, B.pivs, (const zf *)B.blks, B.scales, B.coef, B.ops, B.norms, 2 * maxit, tol,
-                         nearest ? 1 : 0, ev2, B.evec, rs2, ff2, B.iters, B.npadmax, B.nymax);
-    else
-      hipLaunchKernelGGL((k_invit2<double, 16, 256>), dim3(ns), dim3(256), lds2, c->stream, B.pts, B.sel,
-                         (const zd *)B.mats, B.pivs, (const zd *)B.blks, B.scales, B.coef, B.ops, B.norms, 2 * maxit, tol,
-                         nearest ? 1 : 0, ev2, B.evec, rs2, ff2, B.iters, B.npadmax, B.nymax);
-    HIPCHECK(c, hipGetLastError());
-    HIPCHECK(c, hipEventRecord(c->ev[6], c->stream));
-    std::vector<zd> e2(2 * (size_t)npts);
-    std::vector<double> r2(2 * (size_t)npts), f2(2 * (size_t)npts);
-    HIPCHECK(c, hipMemcpyAsync(e2.data(), ev2, sizeof(zd) * 2 * npts, hipMemcpyDeviceToHost, c->stream));
-    HIPCHECK(c, hipMemcpyAsync(r2.data(), rs2, sizeof(double) * 2 * npts, hipMemcpyDeviceToHost, c->stream));
-    HIPCHECK(c, hipMemcpyAsync(f2.data(), ff2, sizeof(double) * 2 * npts, hipMemcpyDeviceToHost, c->stream));
-    HIPCHECK(c, hipMemcpyAsync(it.data(), B.iters, sizeof(int) * npts, hipMemcpyDeviceToHost, c->stream));
-    HIPCHECK(c, hipStreamSynchronize(c->stream));
-    ms2 = elapsed(c->ev[5], c->ev[6]);
+  if (do2 && !sub.empty()) {
     for (int i : sub) {
       const zd s0 = mk(pts[i].shift.re, pts[i].shift.im);
       // rank of a pair: distance to the shift (nearest) or the leading key
@@ -1497,11 +1502,9 @@ static int lead_invit(stab_ctx *c, int npts, const stab_point *pts, int mode, in
       ri.ms_assemble = elapsed(c->ev[1], c->ev[2]);
       ri.ms_lu = elapsed(c->ev[2], c->ev[3]);
       ri.ms_lu_kernel = elapsed(c->ev[2], c->ev[7]);
-      ri.ms_eig = elapsed(c->ev[3], c->ev[4]) + ms2;
-    } else {
-      ri.ms_eig = ms2;
+      ri.ms_eig = elapsed(c->ev[3], c->ev[6]);     // both passes
     }
-    ri.ms_total = elapsed(c->ev[0], c->ev[4]) + ms2;
+    ri.ms_total = elapsed(c->ev[0], c->ev[6]);
     ri.ms_arnoldi = ri.ms_eig;
     ri.cycles = 1;
     ri.nconv = found;

@@ -1183,9 +1183,22 @@ __global__ __launch_bounds__(NT) void k_invit2(const PointDev *__restrict__ pts,
                                                const double *__restrict__ norms, int maxit, double tol, int want,
                                                zd *__restrict__ eval2, zd *__restrict__ evec,
                                                double *__restrict__ resid2, double *__restrict__ ff2,
-                                               int *__restrict__ iters, int npadmax, int nymax) {
+                                               int *__restrict__ iters, int npadmax, int nymax,
+                                               const int *__restrict__ order, const int *__restrict__ info,
+                                               const int *__restrict__ flag, const zd *__restrict__ eval1,
+                                               const double *__restrict__ ff1, int maxit1) {
   extern __shared__ zd lds[];
-  const int p = sub[blockIdx.x];
+  // sub lists the points; without it work-group g takes point order[g] and
+  // runs only when the first pass (maxit1 steps) left that temporal point
+  // unconverged, not discrete (unless want == 1), or converged only after
+  // 3/4 of its steps -- the host applies the same rule to the same data
+  const int p = sub ? sub[blockIdx.x] : order[blockIdx.x];
+  if (!sub) {
+    const PointDev &Q = pts[p];
+    if (info[p] != 0 || Q.itype != 1) return;
+    const bool ok1 = flag[p] == 0 && (want == 1 || lead2_discrete(Q.alpha.x, eval1[p], ff1[p]));
+    if (ok1 && 4 * iters[p] < 3 * (maxit1 + 1)) return;
+  }
   const PointDev &P = pts[p];
   OpLds L(lds, P, npadmax, nymax, 1, npadmax);
   zd *x1 = L.X2 + 5 * nymax;
