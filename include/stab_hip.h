@@ -81,7 +81,13 @@ typedef struct { double re, im; } stab_z;
  *   mattyp  0 constant mu, 1 Sutherland (input.f90:14-32)
  *   Te      edge temperature as input.f90 computes it: T0 as entered when
  *           mattyp = 1 (input.f90:24-25 runs before Ma is read), 0 otherwise
- *   ny, yi, ymax   grid (sgengrid.f90:26-45); yi must be nonzero
+ *   itype   1 Chebyshev temporal (temporal.f90), 2 Chebyshev spatial
+ *           (spatial.f90), 3 finite-difference temporal (fd_temporal.f90, as
+ *           stab.f90:56-60 runs it: wall at node 0, the explicit fourth-order
+ *           rows of stencils.f90, ny >= 10, ymax > 2 yi; wallt and top are
+ *           not read, as fd_temporal does not read them)
+ *   ny, yi, ymax   grid (sgengrid.f90:26-45; ogengrid.f90:17-48 for itype 3);
+ *                  yi must be nonzero
  *   top, wallt     boundary-condition flags (input.f90:57-59)
  *   alpha, beta    temporal wavenumbers; omega, beta for spatial
  *   shift          sigma: a guess of omega (temporal) or alpha (spatial),
@@ -92,7 +98,7 @@ typedef struct { double re, im; } stab_z;
  */
 #define STAB_MAX_NEV 16
 typedef struct {
-  int32_t itype;     /* 1 temporal, 2 spatial */
+  int32_t itype;     /* 1 temporal, 2 spatial, 3 finite-difference temporal */
   int32_t ny;
   int32_t mattyp;
   int32_t wallt;     /* 0 or 2 */
@@ -141,9 +147,10 @@ const char *stab_last_error(const stab_ctx *ctx);
  * reading profile.<ind>. */
 int stab_profile_set(stab_ctx *ctx, int slot, int nym, const double *y, const double *vt);
 
-/* Grid, interpolated mean flow and Chebyshev operators of one point, as the
+/* Grid, interpolated mean flow and derivative operators of one point, as the
  * device builds them (sgengrid.f90:15-45, getmean.f90:95-112, chebyd.f90:2-60,
- * temporal.f90:136).  y/eta/deta/d2eta/vm may be NULL; vm is ny x 5
+ * temporal.f90:136; for itype 3 ogengrid.f90:17-48 and the explicit rows of
+ * fd_temporal.f90:630-890).  y/eta/deta/d2eta/vm may be NULL; vm is ny x 5
  * column-major; d1/d2 are ny x ny column-major or NULL. */
 int stab_grid(stab_ctx *ctx, const stab_point *pt, double *y, double *eta, double *deta,
               double *d2eta, double *vm, double *d1, double *d2);

@@ -23,6 +23,11 @@ Pinning (see tests/test_oracle.py):
                      TStest/README.md and test/space.1 eigenvalues, and the
                      residual of the reference eigenfunction in our pencil
   * sort/select/normalise/format -- the fixture text files above.
+  * finite-difference temporal (a15, tests/test_fd_oracle.py) -- stencil
+                     rows exact on polynomials to their orders, ogengrid
+                     closed form, rows against an independent linearisation,
+                     fourth-order convergence to the thesis/TStest golden
+                     with an isothermal wall row.
 """
 from __future__ import annotations
 
@@ -325,16 +330,22 @@ def derivative_operators(p: Params):
 # ----------------------------------------------------------------------------
 # a6 / a10: per-node 5x5 coefficient tensors
 # ----------------------------------------------------------------------------
-def _mean_state(p: Params, vm, d1, d2, deta, d2eta, spatial: bool):
+def _mean_state(p: Params, vm, d1, d2, deta, d2eta, spatial: bool, raw=None):
     """Mean-field gradients in physical space and material properties
-    (temporal.f90:126-307; spatial.f90:147-352 with curve=0)."""
+    (temporal.f90:126-307; spatial.f90:147-352 with curve=0).  `raw`, when
+    given, holds the eta-space first and second derivatives of vm (the
+    finite-difference path's grad calls, fd_temporal.f90:131-145) in place of
+    d1 @ vm and d2 @ vm."""
     ny = p.ny
     z = np.zeros(ny)
     rhom = vm[:, 0].copy(); u1m = vm[:, 1].copy()
     u2m = z.copy() if spatial else vm[:, 2].copy()
     u3m = vm[:, 3].copy(); tm = vm[:, 4].copy()
-    g2vm = d1 @ vm
-    g22vm = d2 @ vm
+    if raw is None:
+        g2vm = d1 @ vm
+        g22vm = d2 @ vm
+    else:
+        g2vm, g22vm = raw[0].copy(), raw[1].copy()
     g22vm = g22vm * (deta ** 2)[:, None] + g2vm * d2eta[:, None]
     g2vm = g2vm * deta[:, None]
     zero5 = np.zeros((ny, NDOF))
@@ -759,6 +770,7 @@ class Problem:
     dt1: np.ndarray
     dt2: np.ndarray
     T: dict
+    dg: Optional[np.ndarray] = None    # finite-difference path: d/deta of the mean flow (grad.f90)
 
 
 def setup(p: Params, ym, vt, spatial: bool) -> Problem:
@@ -1073,12 +1085,23 @@ FARFIELD_MAX = 0.1
 CONTINUUM_BAND = 0.01
 
 
-def farfield_ratio(vec, ny):
-    """max |x| over the first ceil(ny/8) nodes (5 variables each) / max |x|."""
+def farfield_ratio(vec, ny, nodes=None):
+    """max |x| over the farfield nodes (5 variables each) / max |x|.  The
+    farfield is the first ceil(ny/8) nodes (Chebyshev ordering, eta near 1)
+    unless `nodes` lists others (fd_farfield_nodes)."""
     a = np.abs(np.asarray(vec)[:5 * ny])
-    nf = 5 * ((ny + 7) // 8)
+    if nodes is None:
+        nodes = np.arange((ny + 7) // 8)
     m = a.max()
-    return float(a[:nf].max() / m) if m > 0 else 1.0
+    far = a.reshape(ny, 5)[np.asarray(nodes)]
+    return float(far.max() / m) if m > 0 else 1.0
+
+
+def fd_farfield_nodes(y, ymax):
+    """Farfield of a finite-difference point (wall at node 0): the nodes of
+    the outer half of the domain, y >= ymax/2 (for yi = 2/46, ymax = 50 the
+    Chebyshev window above starts at y = 25.7 too)."""
+    return np.nonzero(np.asarray(y) >= 0.5 * ymax)[0]
 
 
 def is_discrete(itype, ev, ff, alpha_r=0.0, omega_r=0.0):
@@ -1095,14 +1118,14 @@ def is_discrete(itype, ev, ff, alpha_r=0.0, omega_r=0.0):
     return abs(c - 1.0) > CONTINUUM_BAND
 
 
-def leading_mode(itype, w, V, ny, alpha_r=0.0, omega_r=0.0, candidates=None):
+def leading_mode(itype, w, V, ny, alpha_r=0.0, omega_r=0.0, candidates=None, nodes=None):
     """Index of the leading discrete mode among `candidates` (default: all
-    finite, nonzero eigenvalues), or -1."""
+    finite, nonzero eigenvalues), or -1; `nodes` as for farfield_ratio."""
     idx = candidates if candidates is not None else [j for j in range(len(w))
                                                      if np.isfinite(w[j]) and abs(w[j]) > 1e-9]
     best, key = -1, -np.inf
     for j in idx:
-        ff = farfield_ratio(V[:, j], ny)
+        ff = farfield_ratio(V[:, j], ny, nodes)
         if not is_discrete(itype, w[j], ff, alpha_r, omega_r):
             continue
         k = w[j].imag if itype == 1 else -w[j].imag
@@ -1148,6 +1171,176 @@ def read_evec_dat(path):
 # ----------------------------------------------------------------------------
 # convenience: full reference-method runs
 # ----------------------------------------------------------------------------
+# ----------------------------------------------------------------------------
+# a15: finite-difference temporal path (fd_temporal.f90, itype 3)
+# ----------------------------------------------------------------------------
+# Explicit stencils of module `stencils` (stencils.f90:1-128) that fd_temporal
+# uses, in units of 1/deta (first derivative) and 1/deta^2 (second).  The
+# rational ones are written as fractions.  The third-order boundary closures
+# of Carpenter, Gottlieb & Abarbanel (1994) for the fourth-order interior
+# scheme (gg, gh, gi, gj) are fixed by a stability condition as well as by
+# accuracy, so they are the decimals of stencils.f90:56-90.
+FD_GA = (1.0 / 12.0, -2.0 / 3.0, 2.0 / 3.0, -1.0 / 12.0)          # nodes i-2, i-1, i+1, i+2 (:9-12)
+FD_GG = (-1.87603205562073772290017534999, 3.18538322557789286703668141674,
+         -1.81454567943752757247830550034, 0.59165824105260274421658150053,
+         -0.101052068000505624644095417028, 0.0145883364282753087693133500917)     # row 1, nodes 1-6
+FD_GH = (-0.384254232677925402043233692192, -0.290638947767348681070279077173,
+         0.67176478451531541138011989728, 0.071081659837399872713651693126,
+         -0.073630718761724245070378308432, 0.0056774548542830440901194873933)     # row 2, nodes 1-6
+FD_GI = (0.182885278686826206583549109115, -1.08001475417455516433881053795,
+         0.65787289649662525818641772732, 0.1776170486891931456381189546,
+         0.076779836395827558602005515075, -0.0151403060939170046712807681567)     # row 3, nodes 1-6
+FD_GJ = (-0.0341837103365291857798396404953, 0.224829025740103121729831613399,
+         -0.89081233292845396245426338198, 0.165299947710035014782196870489,
+         0.61343955208752529977826815383, -0.078572482272680288056193615249)       # row 4, nodes 1-6
+FD_DA = (-1.0 / 12.0, 4.0 / 3.0, -5.0 / 2.0, 4.0 / 3.0, -1.0 / 12.0)                # (:97-101)
+FD_DB = (11.0 / 12.0, -5.0 / 3.0, 1.0 / 2.0, 1.0 / 3.0, -1.0 / 12.0)                # (:105-109)
+FD_DD = (35.0 / 12.0, -26.0 / 3.0, 19.0 / 2.0, -14.0 / 3.0, 11.0 / 12.0)            # (:120-124)
+
+
+def ogengrid(p: Params):
+    """Uniform eta = (j-1)/(ny-1) with node 0 at the wall and the algebraic
+    map y = aa eta/(bb - eta), aa = ymax yi/(ymax - 2 yi), bb = 1 + aa/ymax
+    (ogengrid.f90:17-48; half the nodes lie below y = yi).  The yi = 0 branch
+    (ogengrid.f90:49-73) reads its parameters from stdin and is not restated.
+    Returns (y, eta, deta, d2eta, dy)."""
+    ny = p.ny
+    if p.yi == 0.0:
+        raise ValueError("yi = 0 selects the interactive Mahesh map (ogengrid.f90:49-73)")
+    if not p.ymax > 2.0 * p.yi:
+        raise ValueError("ogengrid needs ymax > 2 yi")
+    dy = 1.0 / float(ny - 1)
+    aa = p.ymax * p.yi / (p.ymax - 2.0 * p.yi)
+    bb = 1.0 + aa / p.ymax
+    eta = np.arange(ny, dtype=np.float64) * dy
+    y = aa * eta / (bb - eta)
+    deta = (bb - eta) ** 2 / (aa * bb)
+    d2eta = -2.0 * (bb - eta) ** 3 / (aa * bb) ** 2
+    return y, eta, deta, d2eta, dy
+
+
+def fd_grad_matrix(ny: int, dy: float):
+    """d/deta of the mean flow as a matrix: grad.f90:47-85 (the y part of
+    subroutine grad), fourth-order explicit, five-point one-sided and biased
+    rows at both ends, the ones at the far end mirrored and negated."""
+    G = np.zeros((ny, ny))
+    w = 12.0 * dy
+    G[0, 0:5] = np.array([-25.0, 48.0, -36.0, 16.0, -3.0]) / w
+    G[1, 0:5] = np.array([-3.0, -10.0, 18.0, -6.0, 1.0]) / w
+    for i in range(2, ny - 2):
+        G[i, i - 2:i + 3] = np.array([1.0, -8.0, 0.0, 8.0, -1.0]) / w
+    G[ny - 2, ny - 5:ny] = -np.array([1.0, -6.0, 18.0, -10.0, -3.0]) / w
+    G[ny - 1, ny - 5:ny] = -np.array([-3.0, 16.0, -36.0, 48.0, -25.0]) / w
+    return G
+
+
+def fd_operators(ny: int, dy: float):
+    """First and second eta-derivative rows of the finite-difference pencil,
+    as used by fd_temporal.f90:630-890 (node 0 the wall, node ny-1 the top,
+    whose rows are Dirichlet and stay zero here).  Kept as the reference
+    writes them, including the sign pattern of the second-derivative rows at
+    nodes ny-4 and ny-3 (fd_temporal.f90:826-878: the entries right of the
+    diagonal enter with the opposite sign, except the last one of row ny-3)."""
+    if ny < 10:
+        raise ValueError("the finite-difference closures need ny >= 10")
+    D1 = np.zeros((ny, ny))
+    D2 = np.zeros((ny, ny))
+    h1, h2 = 1.0 / dy, 1.0 / (dy * dy)
+    D1[0, 0:6] = np.array(FD_GG) * h1
+    D2[0, 0:5] = np.array(FD_DD) * h2                   # wall energy row (:666-700)
+    D1[1, 0:6] = np.array(FD_GH) * h1
+    D2[1, 0:5] = np.array(FD_DB) * h2                   # :705-733
+    D1[2, 0:6] = np.array(FD_GI) * h1
+    D2[2, 0:5] = np.array(FD_DA) * h2                   # :735-766
+    D1[3, 0:6] = np.array(FD_GJ) * h1
+    D2[3, 1:6] = np.array(FD_DA) * h2                   # :768-793
+    ga = np.array([FD_GA[0], FD_GA[1], 0.0, FD_GA[2], FD_GA[3]])
+    for i in range(4, ny - 4):                          # :795-812
+        D1[i, i - 2:i + 3] = ga * h1
+        D2[i, i - 2:i + 3] = np.array(FD_DA) * h2
+    r = ny - 4                                          # :814-838
+    D1[r, r - 2:r + 4] = -np.array(FD_GJ[::-1]) * h1
+    D2[r, r - 2:r + 3] = np.array(FD_DA) * np.array([1.0, 1.0, 1.0, -1.0, -1.0]) * h2
+    r = ny - 3                                          # :840-864
+    D1[r, r - 3:r + 3] = -np.array(FD_GI[::-1]) * h1
+    D2[r, r - 2:r + 3] = np.array(FD_DA) * np.array([1.0, 1.0, 1.0, -1.0, 1.0]) * h2
+    r = ny - 2                                          # :866-890
+    D1[r, r - 4:r + 2] = -np.array(FD_GH[::-1]) * h1
+    D2[r, r - 3:r + 2] = np.array(FD_DB[::-1]) * h2
+    return D1, D2
+
+
+def fd_setup(p: Params, ym, vt) -> Problem:
+    """Grid, mean flow, derivative rows and coefficient tensors of one
+    finite-difference temporal point (fd_temporal.f90:87-578): the mean-flow
+    derivatives are the grad stencils applied twice in eta, then mapped to y
+    (:131-151); the tensors are those of temporal.f90:309-598 (the two files
+    form them identically)."""
+    if not p.ider:
+        raise ValueError("ider=0 (external derivative files) is outside this path")
+    y, eta, deta, d2eta, dy = ogengrid(p)
+    vm = getmean(y, ym, vt)
+    dg = fd_grad_matrix(p.ny, dy)
+    g2 = dg @ vm
+    g22 = dg @ g2
+    st = _mean_state(p, vm, None, None, deta, d2eta, False, raw=(g2, g22))
+    T = temporal_coefficients(p, st)
+    d1, d2 = fd_operators(p.ny, dy)
+    return Problem(p, y, eta, deta, d2eta, vm, d1, d2, d1, d2, T, dg)
+
+
+def fd_temporal_pencil(pb: Problem):
+    """Dense A0, B0 of the finite-difference temporal problem, A0 x = omega B0 x
+    (fd_temporal.f90:582-890).  Rows: node 0 (wall) continuity (D1 term only)
+    and energy (columns 0-3 the full row, column 4 without the D1 term),
+    wall momentum rows and all top-node rows Dirichlet (A0 zero, B0 = i);
+    every other row Bh D1 - Vyy D2 + Dh on the rows of fd_operators."""
+    p, T = pb.p, pb.T
+    ny = p.ny
+    n = NDOF * ny
+    al, be = p.alpha, p.beta
+    im = 1j
+    Dh = (T["D"] + im * al * T["A"] + im * be * T["C"]
+          + al ** 2 * T["Vxx"] + al * be * T["Vxz"] + be ** 2 * T["Vzz"])
+    Bh = T["B"] - im * al * T["Vxy"] - im * be * T["Vyz"]
+    Bh = Bh * pb.deta[:, None, None] - T["Vyy"] * pb.d2eta[:, None, None]
+    Vyy = T["Vyy"] * (pb.deta ** 2)[:, None, None]
+    d1, d2 = pb.d1, pb.d2
+    A0 = np.zeros((n, n), dtype=np.complex128)
+    B0 = np.zeros((n, n), dtype=np.complex128)
+    for k in range(4):                                    # :606-608
+        B0[k, k] = im
+    B0[4, 0:5] = im * T["G"][0, 4, :]                     # :610-615
+    for i in range(1, ny - 1):                            # :617-624
+        B0[i * 5:(i + 1) * 5, i * 5:(i + 1) * 5] = im * T["G"][i]
+    w = (ny - 1) * 5
+    for k in range(5):                                    # :626-629
+        B0[w + k, w + k] = im
+    # wall continuity row (:633-656)
+    row = (Bh[0][0][None, :] * d1[0][:, None]).astype(np.complex128)
+    row[0, :] += Dh[0][0]
+    A0[0, :] = row.reshape(-1)
+    # wall energy row (:658-700)
+    row = np.zeros((ny, 5), dtype=np.complex128)
+    for q in range(4):
+        row[:, q] = Bh[0][4, q] * d1[0] - Vyy[0][4, q] * d2[0]
+    row[:, 4] = -Vyy[0][4, 4] * d2[0]
+    row[0, :] += Dh[0][4]
+    A0[4, :] = row.reshape(-1)
+    for i in range(1, ny - 1):                            # :703-890
+        A0[i * 5:(i + 1) * 5, :] = _block_rows(Bh, Vyy, d1, d2, Dh, i)
+    return A0, B0
+
+
+def run_fd_temporal(p: Params, ym, vt):
+    """fd_temporal end to end: ZGESV(B0, A0), ZGEEV, sort by Im omega, scale
+    (fd_temporal.f90:972-1048, the same solve as temporal.f90:771-879)."""
+    pb = fd_setup(p, ym, vt)
+    A0, B0 = fd_temporal_pencil(pb)
+    w, v = temporal_solve_reference(A0, B0)
+    return pb, A0, B0, w, v
+
+
 def run_temporal(p: Params, ym, vt):
     pb = setup(p, ym, vt, spatial=False)
     A0, B0 = temporal_pencil(pb)

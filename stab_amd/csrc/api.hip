@@ -32,6 +32,13 @@ struct Profile {
   std::vector<double> data;   // y[nym], vt[5 nym], fdp[5 nym]
 };
 
+// Temporal points: Chebyshev (itype 1) and finite-difference (itype 3; the
+// reference's stab.f90:56-60 runs fd_temporal with itype set to 1).
+inline bool stab_temporal(const stab_point &p) { return p.itype != 2; }
+
+// Operator sets are keyed by ny and the discretisation.
+inline int op_key(const stab_point &p) { return 2 * p.ny + (p.itype == 3 ? 1 : 0); }
+
 }  // namespace
 
 // Fixed device memory handed out by buf_get while an allocation-free entry
@@ -153,9 +160,13 @@ void edge_props(int mattyp, double Te, double Pr, double dm[3], double &rmue, do
 }
 
 int check_point(stab_ctx *c, const stab_point &p, int idx) {
-  if (p.itype != 1 && p.itype != 2) return fail(c, STAB_E_ARG, "point %d: itype %d is not 1 or 2", idx, p.itype);
+  if (p.itype < 1 || p.itype > 3) return fail(c, STAB_E_ARG, "point %d: itype %d is not 1, 2 or 3", idx, p.itype);
   if (p.ny < 4 || p.ny > 256) return fail(c, STAB_E_ARG, "point %d: ny %d outside 4..256", idx, p.ny);
   if (p.itype == 2 && p.ny > 128) return fail(c, STAB_E_ARG, "point %d: spatial ny %d above 128", idx, p.ny);
+  if (p.itype == 3 && p.ny < 10)
+    return fail(c, STAB_E_ARG, "point %d: finite-difference ny %d below 10 (closures of four rows at each end)", idx, p.ny);
+  if (p.itype == 3 && !(p.ymax > 2.0 * p.yi))
+    return fail(c, STAB_E_ARG, "point %d: finite-difference grid needs ymax > 2 yi (ogengrid.f90:21)", idx);
   if (p.mattyp != 0 && p.mattyp != 1) return fail(c, STAB_E_ARG, "point %d: mattyp %d", idx, p.mattyp);
   if (p.wallt != 0 && p.wallt != 2)
     return fail(c, STAB_E_ARG, "point %d: Illegal value of wallt: %d", idx, p.wallt);
@@ -300,13 +311,17 @@ void layout_batch(stab_ctx *c, int npts, const stab_point *pts, size_t elem, int
       poff += (long long)c->profiles[pts[i].profile].data.size();
     }
   }
-  // Chebyshev operators per distinct ny
+  // operator sets per distinct (ny, discretisation): D1, D2, the wall row of
+  // the wall energy equation's second-derivative operator, row sums; the
+  // finite-difference set adds the mean-flow derivative d/deta (grad.f90)
   long long ooff = 0;
   for (int i = 0; i < npts; ++i) {
-    if (!B.op_off.count(pts[i].ny)) {
-      B.op_off[pts[i].ny] = (int)ooff;
-      B.nys.push_back(pts[i].ny);
-      ooff += 2LL * pts[i].ny * pts[i].ny + 4LL * pts[i].ny + 1;   // D1, D2, Dt2 wall row, row sums
+    const int key = op_key(pts[i]);
+    if (!B.op_off.count(key)) {
+      B.op_off[key] = (int)ooff;
+      B.nys.push_back(key);
+      const long long ny = pts[i].ny;
+      ooff += 2LL * ny * ny + 4LL * ny + 1 + (pts[i].itype == 3 ? ny * ny : 0);
     }
   }
   long long coff = 0, moff = 0, voff = 0, pvoff = 0, boff = 0, soff = 0, eoff = 0;
@@ -314,12 +329,12 @@ void layout_batch(stab_ctx *c, int npts, const stab_point *pts, size_t elem, int
     const stab_point &p = pts[i];
     PointDev &d = B.h[i];
     memset(&d, 0, sizeof d);
-    d.itype = p.itype; d.ny = p.ny; d.n = 5 * p.ny; d.npad = round_up(d.n, LU_ALIGN);
+    d.itype = stab_temporal(p) ? 1 : 2; d.fd = (p.itype == 3); d.ny = p.ny; d.n = 5 * p.ny; d.npad = round_up(d.n, LU_ALIGN);
     d.mattyp = p.mattyp; d.wallt = p.wallt; d.top = p.top;
     d.navier = !(p.Re >= 1.0e98 || p.Re == 0.0);
-    d.nev = p.nev; d.nvec = (p.itype == 1) ? d.n : 2 * d.n;
+    d.nev = p.nev; d.nvec = stab_temporal(p) ? d.n : 2 * d.n;
     d.prof = p.profile; d.nym = c->profiles[p.profile].nym;
-    d.op_off = B.op_off[p.ny];
+    d.op_off = B.op_off[op_key(p)];
     d.Ma = p.Ma; d.Re = p.Re; d.Pr = p.Pr; d.Te = p.Te; d.yi = p.yi; d.ymax = p.ymax;
     double dm[3];
     edge_props(p.mattyp, p.Te, p.Pr, dm, d.rmue, d.rlme, d.cone);
@@ -413,11 +428,16 @@ __global__ void k_fill(double *p, size_t n, double v) {
 
 // Operators, node coefficients, norms (and equilibration scales).
 int run_prep(stab_ctx *c, int npts, Batch &B, bool equilibrate) {
-  for (int ny : B.nys) {
-    double *D1 = B.ops + B.op_off[ny];
-    hipLaunchKernelGGL(k_cheb_d1, dim3(1), dim3(256), 0, c->stream, ny, D1);
-    hipLaunchKernelGGL(k_cheb_d2, dim3((ny * ny + 255) / 256), dim3(256), 0, c->stream, ny, D1, D1 + ny * ny,
-                       D1 + 2 * ny * ny);
+  for (int key : B.nys) {
+    const int ny = key / 2;
+    double *D1 = B.ops + B.op_off[key];
+    if (key & 1) {
+      hipLaunchKernelGGL(k_fd_ops, dim3(1), dim3(256), 0, c->stream, ny, D1);
+    } else {
+      hipLaunchKernelGGL(k_cheb_d1, dim3(1), dim3(256), 0, c->stream, ny, D1);
+      hipLaunchKernelGGL(k_cheb_d2, dim3((ny * ny + 255) / 256), dim3(256), 0, c->stream, ny, D1, D1 + ny * ny,
+                         D1 + 2 * ny * ny);
+    }
     hipLaunchKernelGGL(k_cheb_sums, dim3(1), dim3(256), 0, c->stream, ny, D1, D1 + ny * ny, D1 + 2 * ny * ny);
   }
   hipLaunchKernelGGL(k_nodes, dim3(npts), dim3(256), 0, c->stream, B.pts, B.ops, B.prof, B.coef, B.grid);
@@ -925,7 +945,7 @@ int stab_pencil(stab_ctx *c, const stab_point *pt, int which, stab_z *out) {
     case STAB_MAT_SHIFTED: form = FORM_SHIFTED; break;
     default: return fail(c, STAB_E_ARG, "unknown matrix %d", which);
   }
-  if ((pt->itype == 1) != (form == FORM_A0 || form == FORM_B0 || form == FORM_SHIFTED))
+  if (stab_temporal(*pt) != (form == FORM_A0 || form == FORM_B0 || form == FORM_SHIFTED))
     return fail(c, STAB_E_ARG, "matrix %d does not belong to itype %d", which, pt->itype);
   (void)hipSetDevice(c->device);
   Batch B;
@@ -1030,7 +1050,7 @@ int stab_lu_factors(stab_ctx *c, const stab_point *pt, int mode, stab_z *lu, int
 
 size_t stab_evec_bytes(int npts, const stab_point *pts) {
   size_t s = 0;
-  for (int i = 0; i < npts; ++i) s += (size_t)(pts[i].itype == 1 ? 5 : 10) * pts[i].ny * STAB_MAX_NEV;
+  for (int i = 0; i < npts; ++i) s += (size_t)(stab_temporal(pts[i]) ? 5 : 10) * pts[i].ny * STAB_MAX_NEV;
   return s * sizeof(stab_z);
 }
 
@@ -1189,7 +1209,7 @@ int stab_solve(stab_ctx *c, int npts, const stab_point *pts, int mode, stab_z *e
   if (rc != STAB_OK) return rc;
   // offsets of each point's vector block in the caller's layout
   std::vector<size_t> off(npts + 1, 0);
-  for (int i = 0; i < npts; ++i) off[i + 1] = off[i] + (size_t)(pts[i].itype == 1 ? 5 : 10) * pts[i].ny * STAB_MAX_NEV;
+  for (int i = 0; i < npts; ++i) off[i + 1] = off[i] + (size_t)(stab_temporal(pts[i]) ? 5 : 10) * pts[i].ny * STAB_MAX_NEV;
   size_t so = 0;
   for (int k = 0; k < nf; ++k) {
     const int i = idx[k];
@@ -1223,7 +1243,7 @@ int stab_solve(stab_ctx *c, int npts, const stab_point *pts, int mode, stab_z *e
 static bool lead_discrete(const stab_point &p, zd ev, double ff) {
   if (!(ff < STAB_FARFIELD_MAX)) return false;
   double c;
-  if (p.itype == 1) {
+  if (stab_temporal(p)) {
     if (p.alpha.re == 0.0) return true;
     c = ev.x / p.alpha.re;
   } else {
@@ -1234,7 +1254,7 @@ static bool lead_discrete(const stab_point &p, zd ev, double ff) {
 }
 
 // Larger is less stable: Im(omega) (temporal), -Im(alpha) (spatial).
-static double lead_key(const stab_point &p, zd ev) { return p.itype == 1 ? ev.y : -ev.y; }
+static double lead_key(const stab_point &p, zd ev) { return stab_temporal(p) ? ev.y : -ev.y; }
 
 struct LeadOut {
   stab_z *eval;
@@ -1401,7 +1421,7 @@ static int lead_invit(stab_ctx *c, int npts, const stab_point *pts, int mode, in
   if (use_graph) {
     std::vector<long long> sig = {npts, f32, maxit, (long long)lds, B.npadmax, B.nymax, B.nvecmax,
                                   (long long)B.nscales, (long long)lds2, do2, nearest};
-    for (int ny : B.nys) { sig.push_back(ny); sig.push_back(B.op_off[ny]); }
+    for (int key : B.nys) { sig.push_back(key); sig.push_back(B.op_off[key]); }
     for (const void *q : {(const void *)B.pts, (const void *)B.prof, (const void *)B.ops, (const void *)B.coef,
                           (const void *)B.grid, (const void *)B.mats, (const void *)B.pivs, (const void *)B.blks,
                           (const void *)B.scales, (const void *)B.norms, (const void *)B.eval, (const void *)B.evec,
@@ -1457,7 +1477,7 @@ static int lead_invit(stab_ctx *c, int npts, const stab_point *pts, int mode, in
   // pair, not converged, could still outrank it (lead2_competes).
   std::vector<int> sel(npts, 0), sub;
   for (int i = 0; i < npts; ++i)
-    if (!inf[i] && pts[i].itype == 1 && (!ok[i] || 4 * it[i] >= 3 * (maxit + 1))) sub.push_back(i);
+    if (!inf[i] && stab_temporal(pts[i]) && (!ok[i] || 4 * it[i] >= 3 * (maxit + 1))) sub.push_back(i);
   if (do2 && !sub.empty()) {
     for (int i : sub) {
       const zd s0 = mk(pts[i].shift.re, pts[i].shift.im);
@@ -1545,8 +1565,8 @@ int stab_solve_leading(stab_ctx *c, int npts, const stab_point *pts, int mode, i
   std::vector<double> rs(nf), ff(nf);
   std::vector<int32_t> it(nf), inf(nf);
   std::vector<size_t> soff(nf + 1, 0), off(npts + 1, 0);
-  for (int k = 0; k < nf; ++k) soff[k + 1] = soff[k] + (size_t)(sub[k].itype == 1 ? 5 : 10) * sub[k].ny;
-  for (int i = 0; i < npts; ++i) off[i + 1] = off[i] + (size_t)(pts[i].itype == 1 ? 5 : 10) * pts[i].ny;
+  for (int k = 0; k < nf; ++k) soff[k + 1] = soff[k] + (size_t)(stab_temporal(sub[k]) ? 5 : 10) * sub[k].ny;
+  for (int i = 0; i < npts; ++i) off[i + 1] = off[i] + (size_t)(stab_temporal(pts[i]) ? 5 : 10) * pts[i].ny;
   std::vector<stab_z> vec(evec ? soff[nf] : 0);
   const LeadOut fo{ev.data(), evec ? vec.data() : nullptr, rs.data(), ff.data(), it.data(), inf.data()};
   stab_stats fst;

@@ -19,10 +19,10 @@ __device__ inline double row_norm2(const PointDev &P, const zd *node, const doub
     form_coef(P, node, form, p, q, P.shift, bh, vyy, dh);
     zd off;
     double body;
-    if (kind == ROW_GENERIC || (P.itype == 2 && q < 4)) {
+    if (kind == ROW_GENERIC || (!walle_vyy_d1(P) && q < 4)) {
       body = abs2(bh) * s11 + vyy * vyy * s22 - 2.0 * bh.x * vyy * s12;
       off = bh * d1 - mk(vyy * d2, 0.0);
-    } else if (q < 4) {          // temporal wall energy row: (bh - vyy) D1
+    } else if (q < 4) {          // Chebyshev temporal wall energy row: (bh - vyy) D1
       const zd b2 = bh - mk(vyy, 0.0);
       body = abs2(b2) * s11;
       off = b2 * d1;
@@ -92,7 +92,7 @@ __global__ __launch_bounds__(256) void k_rowscale(const PointDev *__restrict__ p
         const double d1 = D1[(size_t)j * ny + i], d2 = D2[(size_t)j * ny + i];
         for (int q = 0; q < 5; ++q) {
           zd v;
-          if (kind == ROW_GENERIC || (P.itype == 2 && q < 4)) v = bh[q] * d1 - mk(vyy[q] * d2, 0.0);
+          if (kind == ROW_GENERIC || (!walle_vyy_d1(P) && q < 4)) v = bh[q] * d1 - mk(vyy[q] * d2, 0.0);
           else if (q < 4) v = bh[q] * d1 - mk(vyy[q] * d1, 0.0);
           else v = mk(-vyy[q] * dt2w[j], 0.0);
           if (j == i) v = v + dh[q];
@@ -170,8 +170,8 @@ __global__ __launch_bounds__(256) void k_assemble_rows(const PointDev *__restric
       zd v;
       if (kind == ROW_GENERIC) {
         v = bh[q] * d1 - mk(vyy[q] * d2, 0.0);
-      } else if (q < 4) {   // wall energy row (wallt = 2)
-        v = (P.itype == 1) ? bh[q] * d1 - mk(vyy[q] * d1, 0.0) : bh[q] * d1 - mk(vyy[q] * d2, 0.0);
+      } else if (q < 4) {   // wall energy row (wallt = 2, or finite differences)
+        v = walle_vyy_d1(P) ? bh[q] * d1 - mk(vyy[q] * d1, 0.0) : bh[q] * d1 - mk(vyy[q] * d2, 0.0);
       } else {
         v = bh[q] * 0.0 - mk(vyy[q] * dt2w[j], 0.0);
       }

@@ -134,8 +134,8 @@ struct PointDev {
   int itype, ny, n, npad;        // n = 5 ny; npad = n rounded up to the LU block
   int mattyp, wallt, top, navier;
   int nev, nvec, prof, nym;      // nvec = n (temporal) or 2n (spatial)
-  int op_off;                    // offset (doubles) of D1/D2/dt2w for this ny
-  int pad0;
+  int op_off;                    // offset (doubles) of this point's operator set
+  int fd;                        // 1: finite-difference temporal point (fd_temporal.f90; itype 3)
   double Ma, Re, Pr, Te, yi, ymax;
   double dm1, dm2, dm3;          // datmat (input.f90:28-35)
   double rmue, rlme, cone;       // edge properties (input.f90:43-44)

@@ -776,18 +776,29 @@ __global__ __launch_bounds__(NT) void k_restart(const PointDev *__restrict__ pts
 // Final Ritz pairs: x_i = V y_i, omega_i = sigma + 1/theta_i, getevec-style
 // normalisation (first entry of largest modulus among the first 5 ny
 // components becomes 1, getevec.f90:179-189), FP64 relative residual.
-// Largest modulus over the nodes of the outer eighth of the domain (node
-// 1 .. ceil(ny/8), eta near 1, the freestream side; all five variables) of a
-// vector normalised to a largest entry of modulus 1.  Discrete modes decay
-// there, the continuous spectrum does not.
+// Whether node i lies in the farfield: the outer eighth of the Chebyshev
+// nodes (node 0 .. ceil(ny/8) - 1, eta near 1); for a finite-difference
+// point, whose node 0 is the wall, the nodes of the outer half of the domain,
+// y >= ymax/2, with y formed as k_nodes forms it (for yi = 2/46, ymax = 50
+// the Chebyshev window starts at y = 25.7 too).
+__device__ inline bool farfield_node(const PointDev &P, int i) {
+  if (!P.fd) return i < (P.ny + 7) / 8;
+  const double dy = 1.0 / (double)(P.ny - 1);
+  const double aa = P.ymax * P.yi / (P.ymax - 2.0 * P.yi), bb = 1.0 + aa / P.ymax;
+  const double eta = (double)i * dy;
+  return aa * eta / (bb - eta) >= 0.5 * P.ymax;
+}
+
+// Largest modulus over the farfield nodes (all five variables) of a vector
+// normalised to a largest entry of modulus 1.  Discrete modes decay there,
+// the continuous spectrum does not.
 template <int NT>
 __device__ double farfield_ratio(const PointDev &P, const zd *x, zd *red) {
-  const int nf = 5 * ((P.ny + 7) / 8);
   double a = 0.0, b = 0.0;
   for (int r = threadIdx.x; r < P.n; r += NT) {
     const double v = abs2(x[r]);
     b = fmax(b, v);
-    if (r < nf) a = fmax(a, v);
+    if (farfield_node(P, r / 5)) a = fmax(a, v);
   }
   for (int off = 32; off > 0; off >>= 1) {
     a = fmax(a, __shfl_xor(a, off));

@@ -9,6 +9,11 @@
 //   * the wall energy row when wallt = 2, which uses D1 in columns 0-3 and the
 //     operators Dt1 (zero wall row) and Dt2 = D1 Dt1 in column 4
 //     (temporal.f90:724-752, spatial.f90:760-788).
+// Finite-difference points (fd_temporal.f90) have the wall at node 0 and
+// explicit rows in D1/D2; their wall momentum rows and all top rows are
+// Dirichlet, and their wall energy row is the generic row in columns 0-3 and
+// -VYY D2(wall, :) in column 4 (fd_temporal.f90:658-700), the form of the
+// wallt = 2 row with D2's wall row in the place of Dt2's.
 #pragma once
 #include "common.h"
 
@@ -17,6 +22,11 @@ enum { ROW_GENERIC = 0, ROW_DIRICHLET = 1, ROW_WALLE = 2, ROW_PAD = 3 };
 __device__ inline int row_kind(const PointDev &P, int r) {
   if (r >= P.n) return ROW_PAD;
   const int i = r / 5, p = r - 5 * (r / 5);
+  if (P.fd) {
+    if (i == P.ny - 1) return ROW_DIRICHLET;                      // fd_temporal.f90:626-629
+    if (i > 0 || p == 0) return ROW_GENERIC;
+    return (p < 4) ? ROW_DIRICHLET : ROW_WALLE;                   // :606-615, :658-700
+  }
   if (i == 0) {
     if (p != 0) return ROW_DIRICHLET;
     if (P.itype == 2 && P.top != 1) return ROW_DIRICHLET;    // spatial.f90:794-798
@@ -29,6 +39,11 @@ __device__ inline int row_kind(const PointDev &P, int r) {
   }
   return ROW_GENERIC;
 }
+
+// Whether the wall energy row of a point takes its VYY term in columns 0-3
+// through D1 (the Chebyshev temporal form, temporal.f90:733-735) rather than
+// through D2 (spatial.f90:767-769, fd_temporal.f90:666-690).
+__device__ inline bool walle_vyy_d1(const PointDev &P) { return P.itype == 1 && !P.fd; }
 
 // Effective (BH, VYY, DH) coefficients of row r = (i,p), column q, for a form.
 struct RowCoef { zd bh, dh; double vyy; bool wall_d1_vyy; };
@@ -94,9 +109,9 @@ __device__ inline zd pencil_entry(const PointDev &P, const zd *__restrict__ coef
   zd v;
   if (kind == ROW_GENERIC) {
     v = bh * d1 - mk(vyy * D2[(size_t)j * ny + i], 0.0);
-  } else {   // ROW_WALLE (wallt = 2)
+  } else {   // ROW_WALLE (wallt = 2, or a finite-difference point)
     if (q < 4) {
-      if (P.itype == 1) v = bh * d1 - mk(vyy * d1, 0.0);                       // temporal.f90:733-735
+      if (walle_vyy_d1(P)) v = bh * d1 - mk(vyy * d1, 0.0);                   // temporal.f90:733-735
       else v = bh * d1 - mk(vyy * D2[(size_t)j * ny + i], 0.0);                // spatial.f90:767-769
     } else {
       v = bh * 0.0 - mk(vyy * dt2w[j], 0.0);                                  // Dt1(ny,:) = 0
@@ -152,8 +167,9 @@ __device__ void pencil_apply(const PointDev &P, const zd *__restrict__ coef, con
     }
   }
   if (need_d1) {
-    if (P.wallt == 2 && t < 64) {
-      // Dt2(wall,:) . x_T  (temporal.f90:137-140): one wave reduces
+    if ((P.wallt == 2 || P.fd) && t < 64) {
+      // Dt2(wall,:) . x_T  (temporal.f90:137-140; D2(wall,:) . x_T for a
+      // finite-difference point): one wave reduces
       zd s = czero<double>();
       for (int j = t; j < ny; j += 64) s = s + x[j * 5 + 4] * dt2w[j];
       for (int off = 32; off > 0; off >>= 1) {
@@ -181,7 +197,7 @@ __device__ void pencil_apply(const PointDev &P, const zd *__restrict__ coef, con
         } else if (kind == ROW_GENERIC) {
           term = bh * X1[i * 5 + q] - X2[i * 5 + q] * vyy;
         } else if (q < 4) {
-          term = (P.itype == 1) ? (bh * X1[i * 5 + q] - X1[i * 5 + q] * vyy) : (bh * X1[i * 5 + q] - X2[i * 5 + q] * vyy);
+          term = walle_vyy_d1(P) ? (bh * X1[i * 5 + q] - X1[i * 5 + q] * vyy) : (bh * X1[i * 5 + q] - X2[i * 5 + q] * vyy);
         } else {
           term = czero<double>() - red[0] * vyy;
         }

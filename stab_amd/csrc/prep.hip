@@ -1,6 +1,7 @@
 // Preparation kernels: spline of the mean-flow profile, Chebyshev operators,
 // grid/mapping, interpolated mean flow and the per-node 5x5 coefficient blocks.
 #include "common.h"
+#include "fd_stencils.h"
 
 // Second derivatives of the cubic spline through one profile column with the
 // cantilever end condition, lambda = 1 (spline.f90:2-49).  One thread per
@@ -97,6 +98,27 @@ __global__ void k_cheb_sums(int ny, const double *__restrict__ D1, const double 
     double s = 0.0;
     for (int j = 0; j < ny; ++j) s += dt2w[j] * dt2w[j];
     sums[3 * ny] = s;
+  }
+}
+
+// Finite-difference operator set for one ny (fd_temporal.f90, itype 3), one
+// block, one thread per row i: the pencil's D1 and D2 rows (the wall row of
+// D2 also in the slot of the wall energy row's second-derivative row), then,
+// after the row sums, the mean-flow derivative d/deta of grad.f90:47-85.
+// Column-major, ld = ny; the row sums are k_cheb_sums' job.
+__global__ void k_fd_ops(int ny, double *__restrict__ ops) {
+  double *D1 = ops, *D2 = D1 + (size_t)ny * ny, *d2w = D2 + (size_t)ny * ny;
+  double *Dg = d2w + 4 * ny + 1;
+  const double dy = 1.0 / (double)(ny - 1);       // ogengrid.f90:22
+  const double h1 = 1.0 / dy, h2 = 1.0 / (dy * dy), w = 12.0 * dy;
+  const int i = threadIdx.x;
+  if (i >= ny) return;
+  for (int j = 0; j < ny; ++j) {
+    const double a = fd::d1_coef(ny, i, j) * h1, b = fd::d2_coef(ny, i, j) * h2;
+    D1[(size_t)j * ny + i] = a;
+    D2[(size_t)j * ny + i] = b;
+    Dg[(size_t)j * ny + i] = fd::g_coef(ny, i, j) / w;
+    if (i == 0) d2w[j] = b;
   }
 }
 
@@ -513,12 +535,22 @@ __global__ __launch_bounds__(256) void k_nodes(const PointDev *__restrict__ pts,
   const PointDev P = pts[blockIdx.x];
   const int ny = P.ny, i = threadIdx.x;
   const bool spatial = (P.itype == 2);
-  __shared__ double s_vm[5][256];
+  __shared__ double s_vm[5][256], s_g[5][256];
   const double *D1 = ops + P.op_off, *D2 = D1 + (size_t)ny * ny;
+  const double *Dg = D2 + (size_t)ny * ny + 4 * ny + 1;     // finite-difference set only
   const double *py = prof + P.prof_off, *pv = py + P.nym, *pf = pv + 5 * (size_t)P.nym;
   double y = 0, eta = 0, deta = 0, d2eta = 0;
   double vm[5] = {0, 0, 0, 0, 0};
-  if (i < ny) {
+  if (i < ny && P.fd) {
+    // ogengrid.f90:17-48 (yi != 0): uniform eta from the wall, y = aa eta / (bb - eta)
+    const double dy = 1.0 / (double)(ny - 1);
+    const double aa = P.ymax * P.yi / (P.ymax - 2.0 * P.yi), bb = 1.0 + aa / P.ymax;
+    eta = (double)i * dy;
+    y = aa * eta / (bb - eta);
+    const double t = bb - eta, ab = aa * bb;
+    deta = t * t / ab;
+    d2eta = -2.0 * (t * t * t) / (ab * ab);
+  } else if (i < ny) {
     // sgengrid.f90:15-45
     const double pi = 3.1415926535897932385;
     const double dth = pi / (double)(ny - 1);
@@ -536,6 +568,8 @@ __global__ __launch_bounds__(256) void k_nodes(const PointDev *__restrict__ pts,
       const double u = ym * yi * (yi + 1.0);
       d2eta = -0.5 * t * t * t / (u * u);
     }
+  }
+  if (i < ny) {
     // getmean.f90:95-112
     const double ymaxm = py[P.nym - 1];
     for (int k = 0; k < 5; ++k) {
@@ -549,17 +583,37 @@ __global__ __launch_bounds__(256) void k_nodes(const PointDev *__restrict__ pts,
     for (int k = 0; k < 5; ++k) g[(4 + k) * 256 + i] = vm[k];
   }
   __syncthreads();
-  if (i >= ny) return;
-  // mean derivatives by collocation, mapped to y (temporal.f90:148-179)
   double g2[5], g22[5];
-  for (int k = 0; k < 5; ++k) {
-    double a = 0.0, b = 0.0;
-    for (int j = 0; j < ny; ++j) {
-      a += D1[(size_t)j * ny + i] * s_vm[k][j];
-      b += D2[(size_t)j * ny + i] * s_vm[k][j];
+  if (P.fd) {
+    // grad applied to the mean flow and again to its derivative, both in
+    // eta, then mapped to y (fd_temporal.f90:131-151)
+    double a[5];
+    for (int k = 0; k < 5; ++k) {
+      a[k] = 0.0;
+      if (i < ny)
+        for (int j = 0; j < ny; ++j) a[k] += Dg[(size_t)j * ny + i] * s_vm[k][j];
+      s_g[k][i] = a[k];
     }
-    g22[k] = b * (deta * deta) + a * d2eta;
-    g2[k] = a * deta;
+    __syncthreads();
+    if (i >= ny) return;
+    for (int k = 0; k < 5; ++k) {
+      double b = 0.0;
+      for (int j = 0; j < ny; ++j) b += Dg[(size_t)j * ny + i] * s_g[k][j];
+      g22[k] = b * (deta * deta) + a[k] * d2eta;
+      g2[k] = a[k] * deta;
+    }
+  } else {
+    if (i >= ny) return;
+    // mean derivatives by collocation, mapped to y (temporal.f90:148-179)
+    for (int k = 0; k < 5; ++k) {
+      double a = 0.0, b = 0.0;
+      for (int j = 0; j < ny; ++j) {
+        a += D1[(size_t)j * ny + i] * s_vm[k][j];
+        b += D2[(size_t)j * ny + i] * s_vm[k][j];
+      }
+      g22[k] = b * (deta * deta) + a * d2eta;
+      g2[k] = a * deta;
+    }
   }
   NodeState s;
   node_state(P, spatial, vm, g2, g22, s);

@@ -5,11 +5,13 @@
 ! Reads the same list-directed stdin sequence as the reference (input.f90:14-128,
 ! then stab.f90:46-55: index, and x for spatial) and dispatches on itype:
 !   1  temporal('evec.dat', ind)      2  spatial('evec.dat', ind)
+!   3  fd_temporal('evec.dat', ind)
 !   7  mtemporal(ind)                 8  mspatial(ind)
-! The finite-difference solvers (itype 3, 4), Stokes waves (5) and the bump
-! receptivity drivers (6, 9) are not part of this build and exit with status 2.
+! The finite-difference spatial solver (itype 4), Stokes waves (5) and the
+! bump receptivity drivers (6, 9) are not part of this build and exit with
+! status 2.
 !
-! One extension: after the reference inputs of itype 1 and 2 an optional line
+! One extension: after the reference inputs of itype 1, 2 and 3 an optional line
 !     shift_r  shift_i  [nev]
 ! gives the shift (the value the reference's getevec -v reads, thesis/TStest/
 ! run.sh:13-17) and the number of pairs (default 8); for itype 7 and 8 the
@@ -34,6 +36,11 @@ program stab_gpu
     read (*, *) x
     call read_shift_line()
     call spatial(name, ind)
+  case (3)
+    read (*, *) ind
+    itype = 1
+    call read_shift_line()
+    call fd_temporal(name, ind)
   case (7)
     itype = 1
     call mtemporal(ind)

@@ -223,6 +223,31 @@ contains
     p%alpha = alpha; p%beta = beta; p%omega = omega; p%shift = shift
   end function point_from_stuff
 
+  !> One temporal point (itype 1 Chebyshev, 3 finite differences) from module
+  !> stuff and profile.<ind>, solved and written to `name`.
+  subroutine solve_temporal_point(name, ind, itp)
+    use stuff
+    character(*), intent(in) :: name
+    integer, intent(in) :: ind, itp
+    type(stab_point) :: pts(1)
+    character(len=80) :: names(1)
+    complex(8) :: sh
+    integer :: nym
+    real(8), allocatable :: ym(:), vt(:,:)
+    if (curve /= 0) then
+      write (*, *) 'curve /= 0 (wall curvature) is outside this path'
+      call exit(2)
+    end if
+    call fetch_profile(ind, nym, ym, vt)
+    call stab_context()
+    call send_profile(0, nym, ym, vt)
+    sh = cmplx(0.35d0 * real(alpha), 0.0d0, kind=8)
+    if (stab_have_shift) sh = stab_shift
+    pts(1) = point_from_stuff(itp, 0, sh, max(1, min(stab_nev, STAB_MAX_NEV)))
+    names(1) = name
+    call solve_and_write(1, pts, names, [ind], [x])
+  end subroutine solve_temporal_point
+
   !> Solve a batch and write one eigensystem file per point.
   subroutine solve_and_write(npts, pts, names, inds, xs)
     integer, intent(in) :: npts
@@ -253,7 +278,9 @@ contains
         end do
       end if
       if (infs(i) /= STAB_INFO_OK) write (*, *) 'stab: point ', i, ' info = ', infs(i)
-      call write_eigensystem(trim(names(i)), inds(i), nyy, int(pts(i)%itype), nvec, int(pts(i)%nev), &
+      ! fd_temporal writes its file with itype 1 (stab.f90:59)
+      call write_eigensystem(trim(names(i)), inds(i), nyy, merge(1, int(pts(i)%itype), pts(i)%itype == 3), &
+                             nvec, int(pts(i)%nev), &
                              evs((i - 1) * STAB_MAX_NEV + 1:(i - 1) * STAB_MAX_NEV + pts(i)%nev), &
                              vecs(off + 1:off + nvec * STAB_MAX_NEV), pts(i)%omega, pts(i)%alpha, pts(i)%beta, &
                              xs(i), y, eta, deta, d2eta, pts(i)%yi, pts(i)%ymax)
@@ -326,29 +353,22 @@ end subroutine input
 !> temporal(name, ind) (temporal.f90:2): eigenpairs of the temporal pencil
 !> at module stuff's (alpha, beta) on profile.<ind>, written to `name`.
 subroutine temporal(name, ind)
-  use stuff
   use stab_path_util
   implicit none
   character(*), intent(in) :: name
   integer, intent(in) :: ind
-  type(stab_point) :: pts(1)
-  character(len=80) :: names(1)
-  complex(8) :: sh
-  integer :: nym
-  real(8), allocatable :: ym(:), vt(:,:)
-  if (curve /= 0) then
-    write (*, *) 'curve /= 0 (wall curvature) is outside this path'
-    call exit(2)
-  end if
-  call fetch_profile(ind, nym, ym, vt)
-  call stab_context()
-  call send_profile(0, nym, ym, vt)
-  sh = cmplx(0.35d0 * real(alpha), 0.0d0, kind=8)
-  if (stab_have_shift) sh = stab_shift
-  pts(1) = point_from_stuff(1, 0, sh, max(1, min(stab_nev, STAB_MAX_NEV)))
-  names(1) = name
-  call solve_and_write(1, pts, names, [ind], [x])
+  call solve_temporal_point(name, ind, 1)
 end subroutine temporal
+
+!> fd_temporal(name, ind) (fd_temporal.f90:2): the same with the
+!> finite-difference discretisation (ogengrid grid, explicit stencils).
+subroutine fd_temporal(name, ind)
+  use stab_path_util
+  implicit none
+  character(*), intent(in) :: name
+  integer, intent(in) :: ind
+  call solve_temporal_point(name, ind, 3)
+end subroutine fd_temporal
 
 !> spatial(name, ind) (spatial.f90:2): eigenpairs of L(alpha) at module
 !> stuff's (omega, beta) on profile.<ind>, written to `name`.

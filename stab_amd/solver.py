@@ -84,14 +84,15 @@ def read_input(text: str) -> Inputs:
     p.ider = int(nxt()[0])
     f = nxt(); p.top, p.wall, p.wallt, p.curve = (int(v) for v in f[:4])
     p.itype = int(nxt()[0])
-    if p.itype == 1:
+    if p.itype in (1, 3):
         f = nxt(); p.alpha = complex(float(f[0]), float(f[1]))
         f = nxt(); p.beta = complex(float(f[0]), float(f[1]))
     elif p.itype == 2:
         f = nxt(); p.omega = complex(float(f[0]), float(f[1]))
         f = nxt(); p.beta = complex(float(f[0]), float(f[1]))
     else:
-        raise ValueError(f"itype {p.itype}: only the Chebyshev temporal (1) and spatial (2) paths are here")
+        raise ValueError(f"itype {p.itype}: only the Chebyshev temporal (1) and spatial (2) and the "
+                         "finite-difference temporal (3) paths are here")
     p.ind = int(nxt()[0])
     if p.itype == 2:
         p.x = float(nxt()[0])
@@ -256,7 +257,7 @@ class Solver:
         if fallback_shifts is not None:
             fb = np.ascontiguousarray(np.asarray(fallback_shifts, dtype=np.complex128))
             assert fb.shape == (npts,)
-        nv = [(5 if p.itype == 1 else 10) * p.ny for p in points]
+        nv = [(10 if p.itype == 2 else 5) * p.ny for p in points]
         vec = np.zeros(sum(nv), dtype=np.complex128) if want_vectors else None
         ip = C.POINTER(C.c_int32)
         self._check(self.lib.stab_solve_leading(
@@ -297,7 +298,7 @@ class Solver:
         if evbuf is not None:
             vecs, off = [], 0
             for p in points:
-                nv = (5 if p.itype == 1 else 10) * p.ny
+                nv = (10 if p.itype == 2 else 5) * p.ny
                 blk = evbuf[off:off + nv * nev].reshape(nev, nv).T
                 vecs.append(blk[:, :p.nev].copy())
                 off += nv * nev
@@ -318,7 +319,8 @@ def write_evec_dat(path: str, inp: Inputs, grid: dict, evals: np.ndarray, evecs:
     computed eigenvalues (the nev nearest the shift), record 5 the matching
     eigenvectors column by column."""
     ndof = 5
-    rec1 = struct.pack("<10i", inp.ind, inp.ny, ndof, inp.itype, inp.ievec, inp.curve, inp.top, inp.wall,
+    itype = 1 if inp.itype == 3 else inp.itype          # stab.f90:59: fd_temporal runs with itype 1
+    rec1 = struct.pack("<10i", inp.ind, inp.ny, ndof, itype, inp.ievec, inp.curve, inp.top, inp.wall,
                        inp.wallt, 1 if inp.ider else 0)
     rec2 = struct.pack("<6d", inp.omega.real, inp.omega.imag, inp.alpha.real, inp.alpha.imag, inp.beta.real,
                        inp.beta.imag) + struct.pack("<3d", inp.Re, inp.Ma, inp.Pr)
@@ -388,7 +390,7 @@ def eigenfunction_text(d: dict, j: int) -> str:
     vec = normalize(d["evec"][:, j], 5 * ny)
     lines = ["# Re = %s, Ma = %s, Pr = %s" % (_e13(d["Re"]), _e13(d["Ma"]), _e13(d["Pr"]))]
     ev = d["evals"][j]
-    om, al = (ev, d["alpha"]) if d["itype"] == 1 else (d["omega"], ev)
+    om, al = (ev, d["alpha"]) if d["itype"] in (1, 3) else (d["omega"], ev)
     be = d["beta"]
     lines.append("# Omega = (%s,%s)" % (_e21(om.real), _e21(om.imag)))
     lines.append("# Alpha = (%s,%s)" % (_e21(al.real), _e21(al.imag)))

@@ -149,23 +149,24 @@ def linear_tensors(jets, Ma, Re, Pr, law, Te, form, h=1e-30):
     return out
 
 
-def interior_operator(T, d1, deta, d2eta, alpha, beta, omega):
-    """Rows of nodes 1..ny-2 of the operator L(alpha, beta, omega) applied to
-    q(y) exp(i(alpha x + beta z - omega t)), assembled directly from the
-    linear equation (d_t -> -i omega, d_x -> i alpha, d_z -> i beta,
-    d_y -> eta' D1, d_yy -> eta'^2 D1 D1 + eta'' D1).  Returns [5 (ny-2), 5 ny]."""
+def interior_operator(T, d1, deta, d2eta, alpha, beta, omega, d2=None, all_rows=False):
+    """Rows of nodes 1..ny-2 (every node with all_rows) of the operator
+    L(alpha, beta, omega) applied to q(y) exp(i(alpha x + beta z - omega t)),
+    assembled directly from the linear equation (d_t -> -i omega,
+    d_x -> i alpha, d_z -> i beta, d_y -> eta' D1, d_yy -> eta'^2 D2 + eta'' D1
+    with D2 = D1 D1 unless given).  Returns [5 (ny-2), 5 ny] ([5 ny, 5 ny])."""
     ny = d1.shape[0]
     Dy = deta[:, None] * d1
-    Dyy = (deta ** 2)[:, None] * (d1 @ d1) + d2eta[:, None] * d1
+    Dyy = (deta ** 2)[:, None] * (d1 @ d1 if d2 is None else d2) + d2eta[:, None] * d1
     ia, ib, iw = 1j * alpha, 1j * beta, -1j * omega
     diag = (T["D"] + iw * T["G"] + ia * T["A"] + ib * T["C"]
             - (ia * ia) * T["Vxx"] - (ia * ib) * T["Vxz"] - (ib * ib) * T["Vzz"])
     dy = T["B"] - ia * T["Vxy"] - ib * T["Vyz"]
     dyy = -T["Vyy"]
     L = np.zeros((5 * ny, 5 * ny), dtype=np.complex128)
-    for i in range(1, ny - 1):
+    for i in (range(ny) if all_rows else range(1, ny - 1)):
         for p in range(5):
             for q in range(5):
                 L[5 * i + p, q::5] += dy[i, p, q] * Dy[i] + dyy[i, p, q] * Dyy[i]
                 L[5 * i + p, 5 * i + q] += diag[i, p, q]
-    return L[5:5 * (ny - 1)]
+    return L if all_rows else L[5:5 * (ny - 1)]

@@ -65,6 +65,32 @@ def test_driver_reproduces_thesis_goldens(tmp_path, case, ref, key):
 
 
 @pytest.mark.gpu
+def test_driver_fd_temporal(tmp_path):
+    """itype 3 (stab.f90:56-60): thesis/TStest/fd_temporal.inp through
+    fd_temporal on the GPU; evec.dat carries itype 1 as the reference's does,
+    the ogengrid grid (wall first) and the oracle's ZGEEV eigenvalue nearest
+    the guess, and getevec's table lists the nodes from the top down."""
+    from oracle import stab_oracle as so
+    from stab_amd import solver as sv
+    print("fortran driver fd_temporal", flush=True)
+    txt = open(golden("thesis_TStest", "fd_temporal.inp")).read()
+    guess = 1.1467e-1 + 2.35e-3j
+    r = _run(tmp_path, txt, f"{guess.real!r} {guess.imag!r} 4\n")
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    d = sv.read_evec_dat(str(tmp_path / "evec.dat"))
+    assert d["itype"] == 1 and d["ny"] == 128
+    p = so.parse_input(txt)
+    pb, _, _, w, _ = so.run_fd_temporal(p, *so.read_profile(golden("TStest", "profile.0")))
+    assert np.allclose(d["y"], pb.y, rtol=1e-14, atol=0) and d["y"][0] == 0.0
+    j = sv.select_nearest(d["evals"], guess)
+    ref = w[np.argmin(np.abs(w - guess))]
+    assert abs(d["evals"][j] - ref) <= 1e-9 * abs(ref)
+    _, tab = so.parse_table(sv.eigenfunction_text(d, j))
+    assert tab[0, 0] == pytest.approx(50.0, rel=1e-14) and tab[-1, 0] == 0.0
+
+
+@pytest.mark.gpu
 def test_driver_multi_temporal_sweep(tmp_path):
     """itype 7 (mtemporal): a 3 x 2 (alpha, beta) sweep solved in one batched
     call writes eig.1 .. eig.6 (alpha outer, beta inner, mtemporal.f90:25-38)

@@ -1,0 +1,151 @@
+"""GPU parity of the finite-difference temporal path (fd_temporal.f90,
+itype 3) through the C ABI, against the oracle on thesis/TStest's
+fd_temporal.inp (N = 128, yi = 1, ymax = 50) and smaller N.
+
+Tolerances: grid 1e-14 relative, derivative rows bitwise, pencil entries
+1e-13 relative to the matrix max, eigenvalues 1e-9 relative to ZGEEV on the
+oracle's pencil, normalised eigenvectors 1e-6, FP64 residuals 1e-10.
+"""
+import numpy as np
+import pytest
+
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+TS_OMEGA = 1.1467880189410E-001 + 2.3844535276592E-003j      # thesis/TStest/README.md:25
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _text(ny):
+    return open(golden("thesis_TStest", "fd_temporal.inp")).read().replace("128 1.0 50.0", f"{ny} 1.0 50.0")
+
+
+def _point(ny, shift, nev=4):
+    from stab_amd import solver as sv
+    inp = sv.read_input(_text(ny))
+    assert inp.itype == 3
+    return inp, sv.make_point(inp, shift, nev)
+
+
+@pytest.fixture(scope="module")
+def oracle128(ts_profile):
+    from oracle import stab_oracle as so
+    p = so.parse_input(_text(128))
+    return (p,) + so.run_fd_temporal(p, *ts_profile)
+
+
+@pytest.mark.parametrize("ny", [10, 64, 128])
+def test_fd_grid_and_operators(gpu_solver, ts_profile, ny):
+    """ogengrid, the interpolated mean flow and the pencil's D1/D2 rows (the
+    latter bitwise: both sides form coefficient times 1/deta or 1/deta^2)."""
+    from oracle import stab_oracle as so
+    _say(f"fd grid ny={ny}")
+    gpu_solver.set_profile(0, *ts_profile)
+    _, pt = _point(ny, 0.1 + 0j)
+    g = gpu_solver.grid(pt)
+    p = so.parse_input(_text(ny))
+    y, eta, deta, d2eta, dy = so.ogengrid(p)
+    assert np.allclose(g["y"], y, rtol=1e-14, atol=0)
+    assert np.array_equal(g["eta"], eta)
+    assert np.allclose(g["deta"], deta, rtol=1e-14, atol=0)
+    assert np.allclose(g["d2eta"], d2eta, rtol=1e-14, atol=0)
+    assert np.max(np.abs(g["vm"] - so.getmean(y, *ts_profile))) < 1e-13
+    d1, d2 = so.fd_operators(ny, dy)
+    assert np.array_equal(g["d1"], d1)
+    assert np.array_equal(g["d2"], d2)
+
+
+@pytest.mark.parametrize("ny", [16, 128])
+def test_fd_pencil_matches_oracle(gpu_solver, ts_profile, ny):
+    """A0, B0 (fd_temporal.f90:582-890) assembled on the device."""
+    from oracle import stab_oracle as so
+    from stab_amd import _lib
+    _say(f"fd pencil ny={ny}")
+    gpu_solver.set_profile(0, *ts_profile)
+    _, pt = _point(ny, 0.1 + 0j)
+    pb = so.fd_setup(so.parse_input(_text(ny)), *ts_profile)
+    for which, ref in zip((_lib.MAT_A0, _lib.MAT_B0), so.fd_temporal_pencil(pb)):
+        got = gpu_solver.pencil(pt, which)
+        err = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+        _say(f"  matrix {which}: max rel err {err:.2e}")
+        assert err < 1e-13
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_fd_solve_matches_zgeev(gpu_solver, ts_profile, oracle128, mode):
+    """The nev pairs nearest the shift (plain FP64 and mixed FP32-LU paths)
+    equal the oracle's ZGESV + ZGEEV eigenvalues nearest it; the TS wave is
+    among them."""
+    from oracle import stab_oracle as so
+    from stab_amd import solver as sv
+    p, pb, A0, B0, w, v = oracle128
+    _say(f"fd solve N=128 mode {mode}")
+    gpu_solver.set_profile(0, *ts_profile)
+    shift = 0.35 * p.alpha
+    _, pt = _point(128, shift, 6)
+    res = gpu_solver.solve([pt], mode)
+    ev = res.evals[0, :6]
+    _say(f"  info {res.info[0]} evals {ev}")
+    assert res.info[0] == 0
+    fin = np.where(np.isfinite(w) & (np.abs(w) > 1e-9))[0]
+    near = fin[np.argsort(np.abs(w[fin] - shift))[:6]]
+    for k in range(6):
+        j = near[np.argmin(np.abs(w[near] - ev[k]))]
+        assert abs(ev[k] - w[j]) <= 1e-9 * abs(w[j])
+        assert res.resid[0, k] < 1e-10
+        x = so.normalize_evec(v[:, j], 5 * p.ny)
+        assert np.max(np.abs(sv.normalize(res.evecs[0][:, k], 5 * p.ny) - x)) <= 1e-6
+    assert np.min(np.abs(ev - TS_OMEGA)) < 1e-4
+
+
+@pytest.mark.parametrize("method", ["krylov", "invit"])
+def test_fd_leading_mode(gpu_solver, ts_profile, oracle128, method):
+    """stab_solve_leading on a finite-difference point: the least stable
+    discrete mode among the 8 eigenvalues nearest the fallback shift, with the
+    farfield taken over the outer half of the domain (the last nodes)."""
+    from oracle import stab_oracle as so
+    from stab_amd import _lib
+    p, pb, A0, B0, w, v = oracle128
+    _say(f"fd leading {method}")
+    gpu_solver.set_profile(0, *ts_profile)
+    fb = 0.35 * p.alpha
+    fin = np.where(np.isfinite(w) & (np.abs(w) > 1e-9))[0]
+    near = list(fin[np.argsort(np.abs(w[fin] - fb))[:8]])
+    nodes = so.fd_farfield_nodes(pb.y, p.ymax)
+    jl = so.leading_mode(1, w, v, p.ny, p.alpha.real, candidates=near, nodes=nodes)
+    assert abs(w[jl] - TS_OMEGA) < 1e-4
+    _, pt = _point(128, fb, 8)
+    if method == "krylov":
+        out = gpu_solver.solve_leading([pt], _lib.MODE_PLAIN_F64, _lib.LEAD_KRYLOV, want_vectors=True)
+    else:
+        out = gpu_solver.solve_leading([pt], _lib.MODE_MIXED, _lib.LEAD_INVIT, want_vectors=True,
+                                       shifts=[w[jl] * (1.0 + 1e-3)], fallback_shifts=[fb])
+    _say(f"  info {out['info']} eval {out['evals'][0]} ff {out['farfield'][0]:.3e} iters {out['iters']}")
+    assert out["info"][0] == _lib.INFO_OK
+    assert abs(out["evals"][0] - w[jl]) <= 1e-9 * abs(w[jl])
+    assert out["resid"][0] < 1e-10
+    ff = so.farfield_ratio(v[:, jl], p.ny, nodes)
+    assert out["farfield"][0] == pytest.approx(ff, rel=1e-5)
+    assert np.max(np.abs(out["evecs"][0] - so.normalize_evec(v[:, jl], 5 * p.ny))) <= 1e-6
+
+
+def test_fd_and_chebyshev_points_share_a_batch(gpu_solver, ts_profile):
+    """A Chebyshev and a finite-difference point of the same ny in one batch
+    keep their own operator sets: each gives what it gives alone, bitwise."""
+    from stab_amd import solver as sv
+    _say("fd + chebyshev batch")
+    gpu_solver.set_profile(0, *ts_profile)
+    cheb = sv.read_input(open(golden("thesis_TStest", "temporal.inp")).read().replace("96 1.0 0.0", "64 1.0 0.0"))
+    pc = sv.make_point(cheb, TS_OMEGA, 4)
+    _, pf = _point(64, TS_OMEGA, 4)
+    both = gpu_solver.solve([pc, pf], 0)
+    alone = [gpu_solver.solve([q], 0) for q in (pc, pf)]
+    for i in range(2):
+        assert np.array_equal(both.evals[i, :4], alone[i].evals[0, :4])
+        assert np.array_equal(both.evecs[i], alone[i].evecs[0])
+    assert abs(both.evals[0, :4] - TS_OMEGA).min() < 1e-6
+    assert abs(both.evals[1, :4] - TS_OMEGA).min() < 1e-4
