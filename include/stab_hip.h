@@ -82,11 +82,12 @@ typedef struct { double re, im; } stab_z;
  *   Te      edge temperature as input.f90 computes it: T0 as entered when
  *           mattyp = 1 (input.f90:24-25 runs before Ma is read), 0 otherwise
  *   itype   1 Chebyshev temporal (temporal.f90), 2 Chebyshev spatial
- *           (spatial.f90), 3 finite-difference temporal (fd_temporal.f90, as
- *           stab.f90:56-60 runs it: wall at node 0, the explicit fourth-order
- *           rows of stencils.f90, ny >= 10, ymax > 2 yi; wallt and top are
- *           not read, as fd_temporal does not read them)
- *   ny, yi, ymax   grid (sgengrid.f90:26-45; ogengrid.f90:17-48 for itype 3);
+ *           (spatial.f90), 3 / 4 finite-difference temporal / spatial
+ *           (fd_temporal.f90, fd_spatial.f90, as stab.f90:56-67 runs them:
+ *           wall at node 0, the explicit fourth-order rows of stencils.f90,
+ *           ny >= 10, ymax > 2 yi; itype 3 reads neither wallt nor top, as
+ *           fd_temporal does not)
+ *   ny, yi, ymax   grid (sgengrid.f90:26-45; ogengrid.f90:17-48 for itypes 3, 4);
  *                  yi must be nonzero
  *   top, wallt     boundary-condition flags (input.f90:57-59)
  *   alpha, beta    temporal wavenumbers; omega, beta for spatial
@@ -98,7 +99,7 @@ typedef struct { double re, im; } stab_z;
  */
 #define STAB_MAX_NEV 16
 typedef struct {
-  int32_t itype;     /* 1 temporal, 2 spatial, 3 finite-difference temporal */
+  int32_t itype;     /* 1 temporal, 2 spatial, 3 / 4 finite-difference temporal / spatial */
   int32_t ny;
   int32_t mattyp;
   int32_t wallt;     /* 0 or 2 */
@@ -149,8 +150,8 @@ int stab_profile_set(stab_ctx *ctx, int slot, int nym, const double *y, const do
 
 /* Grid, interpolated mean flow and derivative operators of one point, as the
  * device builds them (sgengrid.f90:15-45, getmean.f90:95-112, chebyd.f90:2-60,
- * temporal.f90:136; for itype 3 ogengrid.f90:17-48 and the explicit rows of
- * fd_temporal.f90:630-890).  y/eta/deta/d2eta/vm may be NULL; vm is ny x 5
+ * temporal.f90:136; for itypes 3, 4 ogengrid.f90:17-48 and the explicit rows of
+ * fd_temporal.f90:630-890 / fd_spatial.f90:669-1160).  y/eta/deta/d2eta/vm may be NULL; vm is ny x 5
  * column-major; d1/d2 are ny x ny column-major or NULL. */
 int stab_grid(stab_ctx *ctx, const stab_point *pt, double *y, double *eta, double *deta,
               double *d2eta, double *vm, double *d1, double *d2);

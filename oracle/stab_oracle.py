@@ -1270,12 +1270,35 @@ def fd_operators(ny: int, dy: float):
     return D1, D2
 
 
-def fd_setup(p: Params, ym, vt) -> Problem:
+def fd_spatial_operators(ny: int, dy: float):
+    """The derivative rows of the finite-difference spatial pencil
+    (fd_spatial.f90:669-960): rows 0-3 as fd_operators, the central rows on
+    nodes 4 .. ny-3, row ny-2 the second-to-last row of grad.f90 (D1) with the
+    mirrored biased D2 row, row ny-1 (the top continuity row when top = 1,
+    :881-903) the last row of grad.f90 and no D2."""
+    if ny < 10:
+        raise ValueError("the finite-difference closures need ny >= 10")
+    D1, D2 = fd_operators(ny, dy)
+    h1, h2 = 1.0 / dy, 1.0 / (dy * dy)
+    ga = np.array([FD_GA[0], FD_GA[1], 0.0, FD_GA[2], FD_GA[3]])
+    for i in range(ny - 4, ny - 2):                      # :834-852
+        D1[i, :] = 0.0
+        D2[i, :] = 0.0
+        D1[i, i - 2:i + 3] = ga * h1
+        D2[i, i - 2:i + 3] = np.array(FD_DA) * h2
+    G = fd_grad_matrix(ny, dy)
+    D1[ny - 2, :] = G[ny - 2]                            # :854-879
+    D1[ny - 1, :] = G[ny - 1]                            # :881-903
+    return D1, D2
+
+
+def fd_setup(p: Params, ym, vt, spatial: bool = False) -> Problem:
     """Grid, mean flow, derivative rows and coefficient tensors of one
-    finite-difference temporal point (fd_temporal.f90:87-578): the mean-flow
-    derivatives are the grad stencils applied twice in eta, then mapped to y
-    (:131-151); the tensors are those of temporal.f90:309-598 (the two files
-    form them identically)."""
+    finite-difference point (fd_temporal.f90:87-578, fd_spatial.f90:95-648):
+    the mean-flow derivatives are the grad stencils applied twice in eta, then
+    mapped to y (fd_temporal.f90:131-151); the tensors are those of
+    temporal.f90:309-598 / spatial.f90:354-673 (each pair of files forms them
+    identically)."""
     if not p.ider:
         raise ValueError("ider=0 (external derivative files) is outside this path")
     y, eta, deta, d2eta, dy = ogengrid(p)
@@ -1283,9 +1306,9 @@ def fd_setup(p: Params, ym, vt) -> Problem:
     dg = fd_grad_matrix(p.ny, dy)
     g2 = dg @ vm
     g22 = dg @ g2
-    st = _mean_state(p, vm, None, None, deta, d2eta, False, raw=(g2, g22))
-    T = temporal_coefficients(p, st)
-    d1, d2 = fd_operators(p.ny, dy)
+    st = _mean_state(p, vm, None, None, deta, d2eta, spatial, raw=(g2, g22))
+    T = spatial_coefficients(p, st) if spatial else temporal_coefficients(p, st)
+    d1, d2 = fd_spatial_operators(p.ny, dy) if spatial else fd_operators(p.ny, dy)
     return Problem(p, y, eta, deta, d2eta, vm, d1, d2, d1, d2, T, dg)
 
 
@@ -1330,6 +1353,97 @@ def fd_temporal_pencil(pb: Problem):
     for i in range(1, ny - 1):                            # :703-890
         A0[i * 5:(i + 1) * 5, :] = _block_rows(Bh, Vyy, d1, d2, Dh, i)
     return A0, B0
+
+
+def fd_spatial_pencil(pb: Problem):
+    """C0, C1, C2 of the finite-difference spatial problem (fd_spatial.f90:650-
+    1190), L(alpha) = C0 + alpha C1 + alpha^2 C2.  Rows: wall continuity (D1
+    term only); wall energy with wallt = 2 as in fd_temporal (columns 0-3 the
+    full row, column 4 without the D1 term; C1 column 4 Dh1 only), Dirichlet
+    (C0 = -1) with wallt = 0; wall momentum rows Dirichlet; top continuity
+    generic on the last grad row when top = 1, else Dirichlet, the other top
+    rows Dirichlet; every other row Bh D1 - Vyy D2 + Dh.  The wall energy row
+    takes its time term with G of the node the assembly loops ended on (node
+    ny-2, or ny-1 with top = 1: :883-904 reuse the loop index i)."""
+    p, T = pb.p, pb.T
+    ny = p.ny
+    n = NDOF * ny
+    im = 1j
+    be, om = p.beta, p.omega
+    Dh = T["D"] + im * be * T["C"] + be ** 2 * T["Vzz"]
+    Bh = (T["B"] - im * be * T["Vyz"]) * pb.deta[:, None, None] - T["Vyy"] * pb.d2eta[:, None, None]
+    Vyy = T["Vyy"] * (pb.deta ** 2)[:, None, None]
+    d1, d2 = pb.d1, pb.d2
+    C0 = np.zeros((n, n), dtype=np.complex128)
+    C1 = np.zeros((n, n), dtype=np.complex128)
+    C2 = np.zeros((n, n), dtype=np.complex128)
+    z = np.zeros_like(Vyy)
+
+    def cont_row(M, Bhm, Dhm, i):
+        row = (Bhm[i][0][None, :] * d1[i][:, None]).astype(np.complex128)
+        row[i, :] += Dhm[i][0]
+        M[i * 5, :] = row.reshape(-1)
+
+    def wall_energy(M, Bhm, Vyym, Dhm):
+        row = np.zeros((ny, 5), dtype=np.complex128)
+        for q in range(4):
+            row[:, q] = Bhm[0][4, q] * d1[0] - Vyym[0][4, q] * d2[0]
+        row[:, 4] = -Vyym[0][4, 4] * d2[0]
+        row[0, :] += Dhm[0][4]
+        M[4, :] = row.reshape(-1)
+
+    if p.wallt not in (0, 2):
+        raise ValueError(f"Illegal value of wallt: {p.wallt}")
+    # C0 (:669-923)
+    cont_row(C0, Bh, Dh, 0)
+    if p.wallt == 2:
+        wall_energy(C0, Bh, Vyy, Dh)
+    for i in range(1, ny - 1):
+        C0[i * 5:(i + 1) * 5, :] = _block_rows(Bh, Vyy, d1, d2, Dh, i)
+    if p.top == 1:
+        cont_row(C0, Bh, Dh, ny - 1)
+    C0[0, 0] -= im * om
+    for k in range(1, 4):
+        C0[k, k] = -1.0
+    if p.wallt == 0:
+        C0[4, 4] = -1.0
+    else:
+        iq = ny - 1 if p.top == 1 else ny - 2
+        C0[4, 0:5] -= im * om * T["G"][iq, 4, :]
+    for i in range(1, ny - 1):
+        C0[i * 5:(i + 1) * 5, i * 5:(i + 1) * 5] -= im * om * T["G"][i]
+    w = (ny - 1) * 5
+    if p.top == 1:
+        C0[w, w] -= im * om
+    else:
+        C0[w, w] = -1.0
+    for k in range(1, 5):
+        C0[w + k, w + k] = -1.0
+    # C1 (:951-1160)
+    Dh1 = im * T["A"] + be * T["Vxz"]
+    Bh1 = (-im * T["Vxy"]) * pb.deta[:, None, None]
+    cont_row(C1, Bh1, Dh1, 0)
+    if p.wallt == 2:
+        wall_energy(C1, Bh1, z, Dh1)
+    for i in range(1, ny - 1):
+        C1[i * 5:(i + 1) * 5, :] = _block_rows(Bh1, z, d1, d2, Dh1, i)
+    if p.top == 1:
+        cont_row(C1, Bh1, Dh1, ny - 1)
+    # C2 (:1162-1180)
+    if p.wallt == 2:
+        C2[4, 0:5] = T["Vxx"][0, 4, :]
+    for i in range(1, ny - 1):
+        C2[i * 5:(i + 1) * 5, i * 5:(i + 1) * 5] = T["Vxx"][i]
+    return C0, C1, C2
+
+
+def run_fd_spatial(p: Params, ym, vt):
+    """fd_spatial end to end: the companion of spatial.f90 (ZGETRF, ZGETRS,
+    ZGEEV, alpha = 1/lambda, sort by Im alpha; fd_spatial.f90:1192-1290)."""
+    pb = fd_setup(p, ym, vt, spatial=True)
+    C = fd_spatial_pencil(pb)
+    alp, v = spatial_solve_reference(*C, p.ievec)
+    return pb, C, alp, v
 
 
 def run_fd_temporal(p: Params, ym, vt):

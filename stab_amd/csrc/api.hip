@@ -32,12 +32,15 @@ struct Profile {
   std::vector<double> data;   // y[nym], vt[5 nym], fdp[5 nym]
 };
 
-// Temporal points: Chebyshev (itype 1) and finite-difference (itype 3; the
-// reference's stab.f90:56-60 runs fd_temporal with itype set to 1).
-inline bool stab_temporal(const stab_point &p) { return p.itype != 2; }
+// Temporal points: Chebyshev (itype 1) and finite-difference (itype 3);
+// spatial points: Chebyshev (2) and finite-difference (4).  The reference's
+// stab.f90:56-67 runs fd_temporal / fd_spatial with itype set to 1 / 2.
+inline bool stab_temporal(const stab_point &p) { return p.itype == 1 || p.itype == 3; }
+inline bool stab_fd(const stab_point &p) { return p.itype == 3 || p.itype == 4; }
 
-// Operator sets are keyed by ny and the discretisation.
-inline int op_key(const stab_point &p) { return 2 * p.ny + (p.itype == 3 ? 1 : 0); }
+// Operator sets are keyed by ny and the discretisation: 0 Chebyshev, 1 the
+// finite-difference rows of fd_temporal, 2 those of fd_spatial.
+inline int op_key(const stab_point &p) { return 3 * p.ny + (p.itype == 3 ? 1 : p.itype == 4 ? 2 : 0); }
 
 }  // namespace
 
@@ -160,12 +163,12 @@ void edge_props(int mattyp, double Te, double Pr, double dm[3], double &rmue, do
 }
 
 int check_point(stab_ctx *c, const stab_point &p, int idx) {
-  if (p.itype < 1 || p.itype > 3) return fail(c, STAB_E_ARG, "point %d: itype %d is not 1, 2 or 3", idx, p.itype);
+  if (p.itype < 1 || p.itype > 4) return fail(c, STAB_E_ARG, "point %d: itype %d is not 1 to 4", idx, p.itype);
   if (p.ny < 4 || p.ny > 256) return fail(c, STAB_E_ARG, "point %d: ny %d outside 4..256", idx, p.ny);
-  if (p.itype == 2 && p.ny > 128) return fail(c, STAB_E_ARG, "point %d: spatial ny %d above 128", idx, p.ny);
-  if (p.itype == 3 && p.ny < 10)
+  if (!stab_temporal(p) && p.ny > 128) return fail(c, STAB_E_ARG, "point %d: spatial ny %d above 128", idx, p.ny);
+  if (stab_fd(p) && p.ny < 10)
     return fail(c, STAB_E_ARG, "point %d: finite-difference ny %d below 10 (closures of four rows at each end)", idx, p.ny);
-  if (p.itype == 3 && !(p.ymax > 2.0 * p.yi))
+  if (stab_fd(p) && !(p.ymax > 2.0 * p.yi))
     return fail(c, STAB_E_ARG, "point %d: finite-difference grid needs ymax > 2 yi (ogengrid.f90:21)", idx);
   if (p.mattyp != 0 && p.mattyp != 1) return fail(c, STAB_E_ARG, "point %d: mattyp %d", idx, p.mattyp);
   if (p.wallt != 0 && p.wallt != 2)
@@ -321,7 +324,7 @@ void layout_batch(stab_ctx *c, int npts, const stab_point *pts, size_t elem, int
       B.op_off[key] = (int)ooff;
       B.nys.push_back(key);
       const long long ny = pts[i].ny;
-      ooff += 2LL * ny * ny + 4LL * ny + 1 + (pts[i].itype == 3 ? ny * ny : 0);
+      ooff += 2LL * ny * ny + 4LL * ny + 1 + (stab_fd(pts[i]) ? ny * ny : 0);
     }
   }
   long long coff = 0, moff = 0, voff = 0, pvoff = 0, boff = 0, soff = 0, eoff = 0;
@@ -329,7 +332,7 @@ void layout_batch(stab_ctx *c, int npts, const stab_point *pts, size_t elem, int
     const stab_point &p = pts[i];
     PointDev &d = B.h[i];
     memset(&d, 0, sizeof d);
-    d.itype = stab_temporal(p) ? 1 : 2; d.fd = (p.itype == 3); d.ny = p.ny; d.n = 5 * p.ny; d.npad = round_up(d.n, LU_ALIGN);
+    d.itype = stab_temporal(p) ? 1 : 2; d.fd = stab_fd(p); d.ny = p.ny; d.n = 5 * p.ny; d.npad = round_up(d.n, LU_ALIGN);
     d.mattyp = p.mattyp; d.wallt = p.wallt; d.top = p.top;
     d.navier = !(p.Re >= 1.0e98 || p.Re == 0.0);
     d.nev = p.nev; d.nvec = stab_temporal(p) ? d.n : 2 * d.n;
@@ -429,10 +432,10 @@ __global__ void k_fill(double *p, size_t n, double v) {
 // Operators, node coefficients, norms (and equilibration scales).
 int run_prep(stab_ctx *c, int npts, Batch &B, bool equilibrate) {
   for (int key : B.nys) {
-    const int ny = key / 2;
+    const int ny = key / 3, disc = key % 3;
     double *D1 = B.ops + B.op_off[key];
-    if (key & 1) {
-      hipLaunchKernelGGL(k_fd_ops, dim3(1), dim3(256), 0, c->stream, ny, D1);
+    if (disc) {
+      hipLaunchKernelGGL(k_fd_ops, dim3(1), dim3(256), 0, c->stream, ny, D1, disc == 2 ? 1 : 0);
     } else {
       hipLaunchKernelGGL(k_cheb_d1, dim3(1), dim3(256), 0, c->stream, ny, D1);
       hipLaunchKernelGGL(k_cheb_d2, dim3((ny * ny + 255) / 256), dim3(256), 0, c->stream, ny, D1, D1 + ny * ny,

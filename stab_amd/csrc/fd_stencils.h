@@ -1,8 +1,8 @@
-// Explicit finite-difference rows of the finite-difference temporal path
-// (fd_temporal.f90, itype 3) on the uniform eta grid, node 0 the wall and
-// node ny-1 the top.  Host and device code.
+// Explicit finite-difference rows of the finite-difference solves
+// (fd_temporal.f90, itype 3; fd_spatial.f90, itype 4) on the uniform eta
+// grid, node 0 the wall and node ny-1 the top.  Host and device code.
 //
-// Stencils of module `stencils` (stencils.f90:1-128) that fd_temporal uses.
+// Stencils of module `stencils` (stencils.f90:1-128) that the two files use.
 // The rational ones are written as fractions; the third-order boundary
 // closures of Carpenter, Gottlieb & Abarbanel (1994) for the fourth-order
 // interior scheme are fixed by a stability condition as well as by accuracy,
@@ -37,11 +37,13 @@ constexpr double G1[5] = {-3.0, -10.0, 18.0, -6.0, 1.0};
 // Entry (i, j) of the pencil's first-derivative rows, times deta
 // (fd_temporal.f90:633-890): closures on rows 0-3, the central row on rows
 // 4 .. ny-5, rows ny-4 .. ny-2 the closures of rows 3 .. 1 mirrored and
-// negated; the top row (Dirichlet) is zero.
-__host__ __device__ inline double d1_coef(int ny, int i, int j) {
+// negated; the top row (Dirichlet) is zero.  fd_spatial.f90 (spatial =
+// true) runs the central row to row ny-3 (:834-852); its rows ny-2 and ny-1
+// are those of grad (g_coef, :854-903) and zero here.
+__host__ __device__ inline double d1_coef(int ny, int i, int j, bool spatial = false) {
   if (i <= 3) return (j <= 5) ? GC[i][j] : 0.0;
-  if (i >= ny - 1) return 0.0;
-  if (i <= ny - 5) {
+  if (i >= ny - 1 || (spatial && i == ny - 2)) return 0.0;
+  if (i <= ny - 5 || spatial) {
     const int k = j - i;
     return k == -2 ? GA[0] : k == -1 ? GA[1] : k == 1 ? GA[2] : k == 2 ? GA[3] : 0.0;
   }
@@ -54,8 +56,9 @@ __host__ __device__ inline double d1_coef(int ny, int i, int j) {
 // (nodes 1-5 on row 3, 0-4 on row 2), the biased row on rows 1 and ny-2
 // (mirrored), the one-sided row on the wall row; on rows ny-4 and ny-3 the
 // entries right of the diagonal enter with the opposite sign, except the
-// last one of row ny-3 (:826-878).  The top row is zero.
-__host__ __device__ inline double d2_coef(int ny, int i, int j) {
+// last one of row ny-3 (:826-878).  The top row is zero.  fd_spatial.f90
+// (spatial = true) keeps the central row on rows ny-4 and ny-3 (:834-852).
+__host__ __device__ inline double d2_coef(int ny, int i, int j, bool spatial = false) {
   if (i == 0) return (j <= 4) ? DD[j] : 0.0;
   if (i == 1) return (j <= 4) ? DB[j] : 0.0;
   if (i == 2) return (j <= 4) ? DA[j] : 0.0;
@@ -66,8 +69,8 @@ __host__ __device__ inline double d2_coef(int ny, int i, int j) {
   }
   const int k = j - i + 2;                      // 0..4 over nodes i-2..i+2
   if (k < 0 || k > 4) return 0.0;
-  if (i == ny - 4 && k >= 3) return -DA[k];
-  if (i == ny - 3 && k == 3) return -DA[k];
+  if (!spatial && i == ny - 4 && k >= 3) return -DA[k];
+  if (!spatial && i == ny - 3 && k == 3) return -DA[k];
   return DA[k];
 }
 

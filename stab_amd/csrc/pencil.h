@@ -9,11 +9,13 @@
 //   * the wall energy row when wallt = 2, which uses D1 in columns 0-3 and the
 //     operators Dt1 (zero wall row) and Dt2 = D1 Dt1 in column 4
 //     (temporal.f90:724-752, spatial.f90:760-788).
-// Finite-difference points (fd_temporal.f90) have the wall at node 0 and
-// explicit rows in D1/D2; their wall momentum rows and all top rows are
-// Dirichlet, and their wall energy row is the generic row in columns 0-3 and
-// -VYY D2(wall, :) in column 4 (fd_temporal.f90:658-700), the form of the
-// wallt = 2 row with D2's wall row in the place of Dt2's.
+// Finite-difference points (fd_temporal.f90, fd_spatial.f90) have the wall
+// at node 0 and explicit rows in D1/D2; their wall momentum rows are
+// Dirichlet, and their wall energy row (always for temporal, wallt = 2 for
+// spatial) is the generic row in columns 0-3 and -VYY D2(wall, :) in column 4
+// (fd_temporal.f90:658-700, fd_spatial.f90:699-748), the form of the wallt = 2
+// row with D2's wall row in the place of Dt2's.  The top rows are Dirichlet
+// except the spatial continuity row when top = 1 (fd_spatial.f90:881-903).
 #pragma once
 #include "common.h"
 
@@ -23,9 +25,11 @@ __device__ inline int row_kind(const PointDev &P, int r) {
   if (r >= P.n) return ROW_PAD;
   const int i = r / 5, p = r - 5 * (r / 5);
   if (P.fd) {
-    if (i == P.ny - 1) return ROW_DIRICHLET;                      // fd_temporal.f90:626-629
+    if (i == P.ny - 1)                                            // fd_temporal.f90:626-629
+      return (p == 0 && P.itype == 2 && P.top == 1) ? ROW_GENERIC : ROW_DIRICHLET;   // fd_spatial.f90:913-923
     if (i > 0 || p == 0) return ROW_GENERIC;
-    return (p < 4) ? ROW_DIRICHLET : ROW_WALLE;                   // :606-615, :658-700
+    if (p < 4) return ROW_DIRICHLET;                              // :606-615
+    return (P.itype == 1 || P.wallt == 2) ? ROW_WALLE : ROW_DIRICHLET;   // :658-700, fd_spatial.f90:889-898
   }
   if (i == 0) {
     if (p != 0) return ROW_DIRICHLET;

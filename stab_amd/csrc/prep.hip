@@ -101,20 +101,23 @@ __global__ void k_cheb_sums(int ny, const double *__restrict__ D1, const double 
   }
 }
 
-// Finite-difference operator set for one ny (fd_temporal.f90, itype 3), one
-// block, one thread per row i: the pencil's D1 and D2 rows (the wall row of
-// D2 also in the slot of the wall energy row's second-derivative row), then,
-// after the row sums, the mean-flow derivative d/deta of grad.f90:47-85.
-// Column-major, ld = ny; the row sums are k_cheb_sums' job.
-__global__ void k_fd_ops(int ny, double *__restrict__ ops) {
+// Finite-difference operator set for one ny (fd_temporal.f90, itype 3, or
+// fd_spatial.f90, itype 4: `spatial`), one block, one thread per row i: the
+// pencil's D1 and D2 rows (the wall row of D2 also in the slot of the wall
+// energy row's second-derivative row), then, after the row sums, the
+// mean-flow derivative d/deta of grad.f90:47-85.  Column-major, ld = ny; the
+// row sums are k_cheb_sums' job.
+__global__ void k_fd_ops(int ny, double *__restrict__ ops, int spatial) {
   double *D1 = ops, *D2 = D1 + (size_t)ny * ny, *d2w = D2 + (size_t)ny * ny;
   double *Dg = d2w + 4 * ny + 1;
   const double dy = 1.0 / (double)(ny - 1);       // ogengrid.f90:22
   const double h1 = 1.0 / dy, h2 = 1.0 / (dy * dy), w = 12.0 * dy;
   const int i = threadIdx.x;
   if (i >= ny) return;
+  const bool grad_row = spatial && i >= ny - 2;    // fd_spatial.f90:854-903
   for (int j = 0; j < ny; ++j) {
-    const double a = fd::d1_coef(ny, i, j) * h1, b = fd::d2_coef(ny, i, j) * h2;
+    const double a = grad_row ? fd::g_coef(ny, i, j) / w : fd::d1_coef(ny, i, j, spatial) * h1;
+    const double b = fd::d2_coef(ny, i, j, spatial) * h2;
     D1[(size_t)j * ny + i] = a;
     D2[(size_t)j * ny + i] = b;
     Dg[(size_t)j * ny + i] = fd::g_coef(ny, i, j) / w;
@@ -617,6 +620,12 @@ __global__ __launch_bounds__(256) void k_nodes(const PointDev *__restrict__ pts,
   }
   NodeState s;
   node_state(P, spatial, vm, g2, g22, s);
+  // fd_spatial.f90:899-907 subtracts the time term of the wall energy row
+  // (wallt = 2) with G of the node its assembly loops ended on, node ny-2 (or
+  // ny-1 when top = 1), not the wall's: the energy row of the spatial G is
+  // rho e_T (:574), so that term is -i omega rho(that node) in the T column.
+  const bool wall_g_quirk = spatial && P.fd && P.wallt == 2 && i == 0;
+  const double rho_q = s_vm[0][(P.top == 1) ? ny - 1 : ny - 2];
   zd *out = coef + P.coef_off + (size_t)i * NODE_STRIDE;
   const zd I = mk(0.0, 1.0);
   for (int p = 0; p < 5; ++p) {
@@ -645,7 +654,8 @@ __global__ __launch_bounds__(256) void k_nodes(const PointDev *__restrict__ pts,
         dh = mk(r.D[q], 0.0);
         dh = dh + (I * be) * r.C[q];
         dh = dh + (be * be) * r.Vzz[q];
-        dh = dh - (I * om) * r.G[q];                     // time term (spatial.f90:808-809)
+        const double gq = (wall_g_quirk && p == 4) ? (q == 4 ? rho_q : 0.0) : r.G[q];
+        dh = dh - (I * om) * gq;                         // time term (spatial.f90:808-809)
         bh = mk(r.B[q], 0.0);
         bh = bh - (I * be) * r.Vyz[q];
         bh = bh * deta - mk(vyy * d2eta, 0.0);

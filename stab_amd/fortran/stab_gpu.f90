@@ -5,13 +5,12 @@
 ! Reads the same list-directed stdin sequence as the reference (input.f90:14-128,
 ! then stab.f90:46-55: index, and x for spatial) and dispatches on itype:
 !   1  temporal('evec.dat', ind)      2  spatial('evec.dat', ind)
-!   3  fd_temporal('evec.dat', ind)
+!   3  fd_temporal('evec.dat', ind)   4  fd_spatial('evec.dat', ind)
 !   7  mtemporal(ind)                 8  mspatial(ind)
-! The finite-difference spatial solver (itype 4), Stokes waves (5) and the
-! bump receptivity drivers (6, 9) are not part of this build and exit with
-! status 2.
+! Stokes waves (5) and the bump receptivity drivers (6, 9) are not part of
+! this build and exit with status 2.
 !
-! One extension: after the reference inputs of itype 1, 2 and 3 an optional line
+! One extension: after the reference inputs of itype 1 to 4 an optional line
 !     shift_r  shift_i  [nev]
 ! gives the shift (the value the reference's getevec -v reads, thesis/TStest/
 ! run.sh:13-17) and the number of pairs (default 8); for itype 7 and 8 the
@@ -41,6 +40,12 @@ program stab_gpu
     itype = 1
     call read_shift_line()
     call fd_temporal(name, ind)
+  case (4)
+    read (*, *) ind
+    read (*, *) x
+    itype = 2
+    call read_shift_line()
+    call fd_spatial(name, ind)
   case (7)
     itype = 1
     call mtemporal(ind)

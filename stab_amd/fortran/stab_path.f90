@@ -248,6 +248,31 @@ contains
     call solve_and_write(1, pts, names, [ind], [x])
   end subroutine solve_temporal_point
 
+  !> One spatial point (itype 2 Chebyshev, 4 finite differences) from module
+  !> stuff and profile.<ind>, solved and written to `name`.
+  subroutine solve_spatial_point(name, ind, itp)
+    use stuff
+    character(*), intent(in) :: name
+    integer, intent(in) :: ind, itp
+    type(stab_point) :: pts(1)
+    character(len=80) :: names(1)
+    complex(8) :: sh
+    integer :: nym
+    real(8), allocatable :: ym(:), vt(:,:)
+    if (curve /= 0) then
+      write (*, *) 'curve /= 0 (wall curvature) is outside this path'
+      call exit(2)
+    end if
+    call fetch_profile(ind, nym, ym, vt)
+    call stab_context()
+    call send_profile(0, nym, ym, vt)
+    sh = cmplx(real(omega) / 0.35d0, 0.0d0, kind=8)
+    if (stab_have_shift) sh = stab_shift
+    pts(1) = point_from_stuff(itp, 0, sh, max(1, min(stab_nev, STAB_MAX_NEV)))
+    names(1) = name
+    call solve_and_write(1, pts, names, [ind], [x])
+  end subroutine solve_spatial_point
+
   !> Solve a batch and write one eigensystem file per point.
   subroutine solve_and_write(npts, pts, names, inds, xs)
     integer, intent(in) :: npts
@@ -268,7 +293,7 @@ contains
     do i = 1, npts
       nyy = pts(i)%ny
       nvec = 5 * nyy
-      if (pts(i)%itype == 2) nvec = 2 * nvec
+      if (pts(i)%itype == 2 .or. pts(i)%itype == 4) nvec = 2 * nvec
       allocate (y(nyy), eta(nyy), deta(nyy), d2eta(nyy))
       call stab_check(stab_grid(stab_ctx, pts(i), c_loc(y), c_loc(eta), c_loc(deta), c_loc(d2eta), c_null_ptr, &
                                 c_null_ptr, c_null_ptr))
@@ -278,8 +303,9 @@ contains
         end do
       end if
       if (infs(i) /= STAB_INFO_OK) write (*, *) 'stab: point ', i, ' info = ', infs(i)
-      ! fd_temporal writes its file with itype 1 (stab.f90:59)
-      call write_eigensystem(trim(names(i)), inds(i), nyy, merge(1, int(pts(i)%itype), pts(i)%itype == 3), &
+      ! fd_temporal / fd_spatial write their files with itype 1 / 2 (stab.f90:59,66)
+      call write_eigensystem(trim(names(i)), inds(i), nyy, merge(int(pts(i)%itype) - 2, int(pts(i)%itype), &
+                             pts(i)%itype >= 3), &
                              nvec, int(pts(i)%nev), &
                              evs((i - 1) * STAB_MAX_NEV + 1:(i - 1) * STAB_MAX_NEV + pts(i)%nev), &
                              vecs(off + 1:off + nvec * STAB_MAX_NEV), pts(i)%omega, pts(i)%alpha, pts(i)%beta, &
@@ -373,29 +399,22 @@ end subroutine fd_temporal
 !> spatial(name, ind) (spatial.f90:2): eigenpairs of L(alpha) at module
 !> stuff's (omega, beta) on profile.<ind>, written to `name`.
 subroutine spatial(name, ind)
-  use stuff
   use stab_path_util
   implicit none
   character(*), intent(in) :: name
   integer, intent(in) :: ind
-  type(stab_point) :: pts(1)
-  character(len=80) :: names(1)
-  complex(8) :: sh
-  integer :: nym
-  real(8), allocatable :: ym(:), vt(:,:)
-  if (curve /= 0) then
-    write (*, *) 'curve /= 0 (wall curvature) is outside this path'
-    call exit(2)
-  end if
-  call fetch_profile(ind, nym, ym, vt)
-  call stab_context()
-  call send_profile(0, nym, ym, vt)
-  sh = cmplx(real(omega) / 0.35d0, 0.0d0, kind=8)
-  if (stab_have_shift) sh = stab_shift
-  pts(1) = point_from_stuff(2, 0, sh, max(1, min(stab_nev, STAB_MAX_NEV)))
-  names(1) = name
-  call solve_and_write(1, pts, names, [ind], [x])
+  call solve_spatial_point(name, ind, 2)
 end subroutine spatial
+
+!> fd_spatial(name, ind) (fd_spatial.f90:2): the same with the
+!> finite-difference discretisation (ogengrid grid, explicit stencils).
+subroutine fd_spatial(name, ind)
+  use stab_path_util
+  implicit none
+  character(*), intent(in) :: name
+  integer, intent(in) :: ind
+  call solve_spatial_point(name, ind, 4)
+end subroutine fd_spatial
 
 !> mtemporal(ind) (mtemporal.f90:2-50): the (alpha, beta) sweep, every point
 !> in one batched GPU solve, eig.<iver> per point; shift alpha * stab_phase_guess.

@@ -87,14 +87,14 @@ def read_input(text: str) -> Inputs:
     if p.itype in (1, 3):
         f = nxt(); p.alpha = complex(float(f[0]), float(f[1]))
         f = nxt(); p.beta = complex(float(f[0]), float(f[1]))
-    elif p.itype == 2:
+    elif p.itype in (2, 4):
         f = nxt(); p.omega = complex(float(f[0]), float(f[1]))
         f = nxt(); p.beta = complex(float(f[0]), float(f[1]))
     else:
-        raise ValueError(f"itype {p.itype}: only the Chebyshev temporal (1) and spatial (2) and the "
-                         "finite-difference temporal (3) paths are here")
+        raise ValueError(f"itype {p.itype}: only the Chebyshev (1, 2) and finite-difference (3, 4) "
+                         "temporal and spatial paths are here")
     p.ind = int(nxt()[0])
-    if p.itype == 2:
+    if p.itype in (2, 4):
         p.x = float(nxt()[0])
     if p.curve != 0:
         raise ValueError("curve != 0 (wall curvature) is outside this path")
@@ -257,7 +257,7 @@ class Solver:
         if fallback_shifts is not None:
             fb = np.ascontiguousarray(np.asarray(fallback_shifts, dtype=np.complex128))
             assert fb.shape == (npts,)
-        nv = [(10 if p.itype == 2 else 5) * p.ny for p in points]
+        nv = [(10 if p.itype in (2, 4) else 5) * p.ny for p in points]
         vec = np.zeros(sum(nv), dtype=np.complex128) if want_vectors else None
         ip = C.POINTER(C.c_int32)
         self._check(self.lib.stab_solve_leading(
@@ -298,7 +298,7 @@ class Solver:
         if evbuf is not None:
             vecs, off = [], 0
             for p in points:
-                nv = (10 if p.itype == 2 else 5) * p.ny
+                nv = (10 if p.itype in (2, 4) else 5) * p.ny
                 blk = evbuf[off:off + nv * nev].reshape(nev, nv).T
                 vecs.append(blk[:, :p.nev].copy())
                 off += nv * nev
@@ -319,7 +319,7 @@ def write_evec_dat(path: str, inp: Inputs, grid: dict, evals: np.ndarray, evecs:
     computed eigenvalues (the nev nearest the shift), record 5 the matching
     eigenvectors column by column."""
     ndof = 5
-    itype = 1 if inp.itype == 3 else inp.itype          # stab.f90:59: fd_temporal runs with itype 1
+    itype = {3: 1, 4: 2}.get(inp.itype, inp.itype)     # stab.f90:59,66: fd_* run with itype 1 / 2
     rec1 = struct.pack("<10i", inp.ind, inp.ny, ndof, itype, inp.ievec, inp.curve, inp.top, inp.wall,
                        inp.wallt, 1 if inp.ider else 0)
     rec2 = struct.pack("<6d", inp.omega.real, inp.omega.imag, inp.alpha.real, inp.alpha.imag, inp.beta.real,

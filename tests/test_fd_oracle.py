@@ -256,9 +256,10 @@ def test_fd_leading_mode_is_the_ts_wave(ts_profile, ny):
 
 @pytest.mark.parametrize("ny", [10, 11, 37, 128, 256])
 def test_device_stencil_rows_equal_oracle(tmp_path, ny):
-    """stab_amd/csrc/fd_stencils.h (the rows k_fd_ops writes), compiled with
-    g++ and run on the host, gives the oracle's D1, D2 and grad matrices
-    bit for bit, with the same scalings 1/dy, 1/dy^2, 1/(12 dy)."""
+    """stab_amd/csrc/fd_stencils.h (the rows k_fd_ops writes, for both
+    solves), compiled with g++ and run on the host, gives the oracle's D1, D2
+    and grad matrices bit for bit, with the same scalings 1/dy, 1/dy^2,
+    1/(12 dy)."""
     import os
     import shutil
     import subprocess
@@ -277,8 +278,11 @@ int main(int argc, char **argv) {
   const int ny = std::atoi(argv[1]);
   const double dy = 1.0 / (double)(ny - 1), h1 = 1.0 / dy, h2 = 1.0 / (dy * dy), w = 12.0 * dy;
   for (int i = 0; i < ny; ++i)
-    for (int j = 0; j < ny; ++j)
-      std::printf("%a %a %a\n", fd::d1_coef(ny, i, j) * h1, fd::d2_coef(ny, i, j) * h2, fd::g_coef(ny, i, j) / w);
+    for (int j = 0; j < ny; ++j) {
+      const double s1 = (i >= ny - 2) ? fd::g_coef(ny, i, j) / w : fd::d1_coef(ny, i, j, true) * h1;
+      std::printf("%a %a %a %a %a\n", fd::d1_coef(ny, i, j) * h1, fd::d2_coef(ny, i, j) * h2,
+                  fd::g_coef(ny, i, j) / w, s1, fd::d2_coef(ny, i, j, true) * h2);
+    }
   return 0;
 }
 ''')
@@ -289,9 +293,146 @@ int main(int argc, char **argv) {
     out = subprocess.run([str(exe), str(ny)], capture_output=True, text=True)
     assert out.returncode == 0
     vals = np.array([[float.fromhex(t) for t in ln.split()] for ln in out.stdout.split("\n") if ln.strip()])
-    vals = vals.reshape(ny, ny, 3)
+    vals = vals.reshape(ny, ny, 5)
     dy = 1.0 / float(ny - 1)
     d1, d2 = so.fd_operators(ny, dy)
     assert np.array_equal(vals[:, :, 0], d1)
     assert np.array_equal(vals[:, :, 1], d2)
     assert np.array_equal(vals[:, :, 2], so.fd_grad_matrix(ny, dy))
+    s1, s2 = so.fd_spatial_operators(ny, dy)
+    assert np.array_equal(vals[:, :, 3], s1)
+    assert np.array_equal(vals[:, :, 4], s2)
+
+
+# --- finite-difference spatial (fd_spatial.f90, itype 4) ----------------------------
+
+TS_ALPHA = 2.2804739411185E-001 - 6.5163146952733E-003j       # thesis/TStest/README.md:14 (Chebyshev, N = 96)
+
+
+def _fds_params(ny, top=0, wallt=0):
+    txt = open(golden("thesis_TStest", "fd_spatial.inp")).read().replace("128 1.0 50.0", f"{ny} 1.0 50.0")
+    p = so.parse_input(txt)
+    p.top, p.wallt = top, wallt
+    return p
+
+
+def test_fd_spatial_rows_exact_on_polynomials():
+    """fd_spatial.f90:669-903: central rows on nodes 4 .. ny-3 (degree 4 for
+    D1, 5 for D2), the grad rows at ny-2 and ny-1 (degree 4), the Carpenter
+    closures (degree 3), the biased and one-sided D2 rows (degree 4)."""
+    ny, dy = 24, 1.0 / 23.0
+    D1, D2 = so.fd_spatial_operators(ny, dy)
+    assert _moments_ok(D1, list(range(4, ny)), dy, 4, 1) < 1e-13
+    assert _moments_ok(D1, [0, 1, 2, 3], dy, 3, 1) < 1e-13
+    assert _moments_ok(D2, list(range(2, ny - 2)), dy, 5, 2) < 1e-12
+    assert _moments_ok(D2, [0, 1, ny - 2], dy, 4, 2) < 1e-12
+    assert not D2[ny - 1].any()
+
+
+@pytest.fixture(scope="module")
+def fds_problem(ts_profile):
+    ny = 20
+    p = _fds_params(ny, top=1, wallt=2)
+    p.beta = 0.17 - 0.004j
+    p.omega = 0.061 + 0.0023j
+    pb = so.fd_setup(p, *ts_profile, spatial=True)
+    jets = ns.mean_jets(pb.vm, pb.dg, pb.deta, pb.d2eta)
+    Ti = ns.linear_tensors(jets, p.Ma, p.Re, p.Pr, p.mattyp, p.Te, "spatial")
+    return p, pb, Ti
+
+
+def test_fd_spatial_rows_match_linearised_operator(fds_problem):
+    """Interior rows of L(alpha) = C0 + alpha C1 + alpha^2 C2 equal the
+    independently linearised operator; the wall continuity and top
+    continuity (top = 1) rows too; the wall energy row (wallt = 2) too except
+    that its T column has no first-derivative term and its time term takes G
+    of node ny-1 (fd_spatial.f90:899-907, the loop index left at ny); the
+    other wall and top rows are Dirichlet (C0 = -1, C1 = C2 = 0)."""
+    p, pb, Ti = fds_problem
+    ny = p.ny
+    al, be, om = 0.23 + 0.011j, p.beta, p.omega
+    C0, C1, C2 = so.fd_spatial_pencil(pb)
+    Lo = C0 + al * C1 + al * al * C2
+    Lall = ns.interior_operator(Ti, pb.d1, pb.deta, pb.d2eta, al, be, om, d2=pb.d2, all_rows=True)
+    rows = list(range(5, 5 * (ny - 1))) + [0, 5 * (ny - 1)]
+    scale = np.abs(Lo[rows]).max(axis=1, keepdims=True)
+    assert np.max(np.abs(Lo[rows] - Lall[rows]) / scale) < 1e-12
+    bhT = ((Ti["B"][0, 4, 4] - 1j * al * Ti["Vxy"][0, 4, 4] - 1j * be * Ti["Vyz"][0, 4, 4]) * pb.deta[0]
+           - Ti["Vyy"][0, 4, 4] * pb.d2eta[0])
+    row = Lall[4].copy()
+    row[4::5] -= bhT * pb.d1[0]
+    row[0:5] += 1j * om * (Ti["G"][0, 4, :] - Ti["G"][ny - 1, 4, :])
+    assert np.max(np.abs(Lo[4] - row)) / np.abs(Lo[4]).max() < 1e-12
+    for r in (1, 2, 3) + tuple(range(5 * (ny - 1) + 1, 5 * ny)):
+        e = np.zeros(5 * ny, complex)
+        e[r] = -1.0
+        assert np.array_equal(C0[r], e) and not C1[r].any() and not C2[r].any()
+
+
+def _fds_ts_alpha(ts_profile, ny, top, wallt, near):
+    pb = so.fd_setup(_fds_params(ny, top, wallt), *ts_profile, spatial=True)
+    alp, _ = so.spatial_solve_reference(*so.fd_spatial_pencil(pb), 0)
+    return alp[np.argmin(np.abs(alp - near))]
+
+
+def test_fd_spatial_converges_to_chebyshev_golden(ts_profile):
+    """thesis/TStest/fd_spatial.inp (T = 0 at the wall, top = 0): alpha tends
+    to the Chebyshev golden of README.md at fourth order."""
+    e64 = abs(_fds_ts_alpha(ts_profile, 64, 0, 0, TS_ALPHA) - TS_ALPHA)
+    e128 = abs(_fds_ts_alpha(ts_profile, 128, 0, 0, TS_ALPHA) - TS_ALPHA)
+    assert e128 < 1e-7
+    assert e64 / e128 > 12.0
+
+
+def test_fd_spatial_energy_wall_row_converges_to_chebyshev_with_that_row(ts_profile):
+    """wallt = 2: the finite-difference alpha tends to the Chebyshev spatial
+    pencil given the same wall energy row, time term with the top node's G
+    included; with the wall's own G in that term the Chebyshev limit is
+    missed by more than 3e-6."""
+    p = so.parse_input(open(golden("thesis_TStest", "spatial.inp")).read())
+    pb = so.setup(p, *ts_profile, spatial=True)
+    C0, C1, C2 = so.spatial_pencil(pb)
+    T = pb.T
+    be, om = p.beta, p.omega
+    Dh = T["D"] + 1j * be * T["C"] + be ** 2 * T["Vzz"]
+    Bh = (T["B"] - 1j * be * T["Vyz"]) * pb.deta[:, None, None] - T["Vyy"] * pb.d2eta[:, None, None]
+    Vyy = T["Vyy"] * (pb.deta ** 2)[:, None, None]
+    Dh1 = 1j * T["A"] + be * T["Vxz"]
+    Bh1 = -1j * T["Vxy"] * pb.deta[:, None, None]
+    i = p.ny - 1
+    w = 5 * i
+    r0 = np.zeros((p.ny, 5), complex)
+    r1 = np.zeros((p.ny, 5), complex)
+    for q in range(4):
+        r0[:, q] = Bh[i][4, q] * pb.d1[i] - Vyy[i][4, q] * pb.d2[i]
+        r1[:, q] = Bh1[i][4, q] * pb.d1[i]
+    r0[:, 4] = -Vyy[i][4, 4] * pb.d2[i]
+    r0[i, :] += Dh[i][4] - 1j * om * T["G"][0, 4, :]
+    r1[i, :] += Dh1[i][4]
+    C0[w + 4], C1[w + 4] = r0.reshape(-1), r1.reshape(-1)
+    C2[w + 4] = 0.0
+    C2[w + 4, w:w + 5] = T["Vxx"][i, 4, :]
+    alp, _ = so.spatial_solve_reference(C0, C1, C2, 0)
+    cheb = alp[np.argmin(np.abs(alp - TS_ALPHA))]
+    e64 = abs(_fds_ts_alpha(ts_profile, 64, 0, 2, cheb) - cheb)
+    e128 = abs(_fds_ts_alpha(ts_profile, 128, 0, 2, cheb) - cheb)
+    assert e128 < 5e-7
+    assert e64 / e128 > 4.0
+    fp = _fds_params(128, 0, 2)
+    fpb = so.fd_setup(fp, *ts_profile, spatial=True)
+    D0, D1_, D2_ = so.fd_spatial_pencil(fpb)
+    D0[4, 0:5] += 1j * fp.omega * (fpb.T["G"][fp.ny - 2, 4, :] - fpb.T["G"][0, 4, :])
+    alt, _ = so.spatial_solve_reference(D0, D1_, D2_, 0)
+    assert abs(alt[np.argmin(np.abs(alt - cheb))] - cheb) > 3e-6
+
+
+def test_fd_spatial_leading_mode_is_the_ts_wave(ts_profile):
+    """The TS wave is the least stable discrete spatial mode among the eight
+    alphas nearest omega / 0.35, farfield over the outer half of the domain."""
+    p = _fds_params(96)
+    pb, _, alp, v = so.run_fd_spatial(p, *ts_profile)
+    nodes = so.fd_farfield_nodes(pb.y, p.ymax)
+    fin = np.where(np.isfinite(alp) & (np.abs(alp) > 1e-9))[0]
+    near = list(fin[np.argsort(np.abs(alp[fin] - p.omega / 0.35))[:8]])
+    j = so.leading_mode(2, alp, v[:5 * p.ny], p.ny, omega_r=p.omega.real, candidates=near, nodes=nodes)
+    assert abs(alp[j] - TS_ALPHA) < 1e-6

@@ -91,6 +91,28 @@ def test_driver_fd_temporal(tmp_path):
 
 
 @pytest.mark.gpu
+def test_driver_fd_spatial(tmp_path):
+    """itype 4 (stab.f90:61-67): thesis/TStest/fd_spatial.inp through
+    fd_spatial on the GPU; evec.dat carries itype 2 as the reference's does
+    and the oracle's companion eigenvalue nearest the guess."""
+    from oracle import stab_oracle as so
+    from stab_amd import solver as sv
+    print("fortran driver fd_spatial", flush=True)
+    txt = open(golden("thesis_TStest", "fd_spatial.inp")).read().replace("128 1.0 50.0", "96 1.0 50.0")
+    guess = 2.2805e-1 - 6.5e-3j
+    r = _run(tmp_path, txt, f"{guess.real!r} {guess.imag!r} 1\n")
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    d = sv.read_evec_dat(str(tmp_path / "evec.dat"))
+    assert d["itype"] == 2 and d["ny"] == 96 and d["evals"].size == 10 * 96
+    p = so.parse_input(txt)
+    _, _, alp, _ = so.run_fd_spatial(p, *so.read_profile(golden("TStest", "profile.0")))
+    j = sv.select_nearest(d["evals"], guess)
+    ref = alp[np.argmin(np.abs(alp - guess))]
+    assert abs(d["evals"][j] - ref) <= 1e-9 * abs(ref)
+
+
+@pytest.mark.gpu
 def test_driver_multi_temporal_sweep(tmp_path):
     """itype 7 (mtemporal): a 3 x 2 (alpha, beta) sweep solved in one batched
     call writes eig.1 .. eig.6 (alpha outer, beta inner, mtemporal.f90:25-38)

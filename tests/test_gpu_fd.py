@@ -1,6 +1,6 @@
-"""GPU parity of the finite-difference temporal path (fd_temporal.f90,
-itype 3) through the C ABI, against the oracle on thesis/TStest's
-fd_temporal.inp (N = 128, yi = 1, ymax = 50) and smaller N.
+"""GPU parity of the finite-difference paths (fd_temporal.f90, itype 3;
+fd_spatial.f90, itype 4) through the C ABI, against the oracle on thesis/TStest's
+fd_temporal.inp and fd_spatial.inp (yi = 1, ymax = 50) at N = 10 to 128.
 
 Tolerances: grid 1e-14 relative, derivative rows bitwise, pencil entries
 1e-13 relative to the matrix max, eigenvalues 1e-9 relative to ZGEEV on the
@@ -149,3 +149,109 @@ def test_fd_and_chebyshev_points_share_a_batch(gpu_solver, ts_profile):
         assert np.array_equal(both.evecs[i], alone[i].evecs[0])
     assert abs(both.evals[0, :4] - TS_OMEGA).min() < 1e-6
     assert abs(both.evals[1, :4] - TS_OMEGA).min() < 1e-4
+
+
+# --- finite-difference spatial (fd_spatial.f90, itype 4) ----------------------------
+
+TS_ALPHA = 2.2804739411185E-001 - 6.5163146952733E-003j       # thesis/TStest/README.md:14
+
+
+def _stext(ny, top=0, wallt=0):
+    txt = open(golden("thesis_TStest", "fd_spatial.inp")).read().replace("128 1.0 50.0", f"{ny} 1.0 50.0")
+    return txt.replace("0 0 0 0\t", f"{top} 0 {wallt} 0\t")
+
+
+def _spoint(ny, shift, nev=4, top=0, wallt=0):
+    from stab_amd import solver as sv
+    inp = sv.read_input(_stext(ny, top, wallt))
+    assert inp.itype == 4 and inp.top == top and inp.wallt == wallt
+    return inp, sv.make_point(inp, shift, nev)
+
+
+@pytest.mark.parametrize("ny", [10, 64])
+def test_fds_operators(gpu_solver, ts_profile, ny):
+    """The spatial rows (central to node ny-3, grad rows at the top), bitwise."""
+    from oracle import stab_oracle as so
+    _say(f"fd spatial operators ny={ny}")
+    gpu_solver.set_profile(0, *ts_profile)
+    _, pt = _spoint(ny, 0.2 + 0j)
+    g = gpu_solver.grid(pt)
+    d1, d2 = so.fd_spatial_operators(ny, 1.0 / float(ny - 1))
+    assert np.array_equal(g["d1"], d1)
+    assert np.array_equal(g["d2"], d2)
+
+
+@pytest.mark.parametrize("top,wallt", [(0, 0), (1, 2), (0, 2)])
+def test_fds_pencil_matches_oracle(gpu_solver, ts_profile, top, wallt):
+    """C0, C1, C2 (fd_spatial.f90:650-1190) for the wall energy row absent
+    (wallt 0) and present (wallt 2, its time term with G of node ny-1 or ny-2),
+    and the top continuity row absent and present."""
+    from oracle import stab_oracle as so
+    from stab_amd import _lib
+    _say(f"fd spatial pencil top={top} wallt={wallt}")
+    gpu_solver.set_profile(0, *ts_profile)
+    _, pt = _spoint(24, 0.2 + 0j, top=top, wallt=wallt)
+    pb = so.fd_setup(so.parse_input(_stext(24, top, wallt)), *ts_profile, spatial=True)
+    for which, ref in zip((_lib.MAT_C0, _lib.MAT_C1, _lib.MAT_C2), so.fd_spatial_pencil(pb)):
+        got = gpu_solver.pencil(pt, which)
+        err = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+        _say(f"  matrix {which}: max rel err {err:.2e}")
+        assert err < 1e-13
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_fds_solve_matches_zgeev(gpu_solver, ts_profile, mode):
+    """The alpha nearest omega / 0.35 (plain FP64 and mixed FP32-LU), the TS
+    wave, equals the oracle's companion ZGEEV eigenvalue.  One pair: the next
+    ones lie 0.144 from the shift in a cluster of entropy/vorticity pairs that
+    Pr = 1 makes nearly degenerate (distances equal to 1e-5), which no
+    nearest-k window separates."""
+    from oracle import stab_oracle as so
+    from stab_amd import solver as sv
+    _say(f"fd spatial solve N=96 mode {mode}")
+    p = so.parse_input(_stext(96))
+    pb, _, alp, v = so.run_fd_spatial(p, *ts_profile)
+    gpu_solver.set_profile(0, *ts_profile)
+    shift = p.omega / 0.35
+    _, pt = _spoint(96, shift, 1)
+    res = gpu_solver.solve([pt], mode)
+    ev = res.evals[0, :1]
+    _say(f"  info {res.info[0]} evals {ev} resid {res.resid[0, :1]}")
+    assert res.info[0] == 0
+    fin = np.where(np.isfinite(alp) & (np.abs(alp) > 1e-9))[0]
+    near = fin[np.argsort(np.abs(alp[fin] - shift))[:1]]
+    for k in range(1):
+        j = near[np.argmin(np.abs(alp[near] - ev[k]))]
+        assert abs(ev[k] - alp[j]) <= 1e-9 * abs(alp[j])
+        assert res.resid[0, k] < 1e-10
+        x = so.normalize_evec(v[:, j], 5 * p.ny)
+        assert np.max(np.abs(sv.normalize(res.evecs[0][:, k], 5 * p.ny) - x)) <= 1e-6
+    assert np.min(np.abs(ev - TS_ALPHA)) < 1e-6
+
+
+def test_fds_leading_mode(gpu_solver, ts_profile):
+    """stab_solve_leading (Krylov-Schur, two pairs) on a finite-difference
+    spatial point returns the TS wave, the oracle's leading discrete mode
+    among the two alphas nearest the shift, farfield over the outer half of
+    the domain; inverse iteration from a perturbed shift reaches it too."""
+    from oracle import stab_oracle as so
+    from stab_amd import _lib
+    _say("fd spatial leading")
+    p = so.parse_input(_stext(96))
+    pb, _, alp, v = so.run_fd_spatial(p, *ts_profile)
+    shift = p.omega / 0.35
+    fin = np.where(np.isfinite(alp) & (np.abs(alp) > 1e-9))[0]
+    near = list(fin[np.argsort(np.abs(alp[fin] - shift))[:2]])
+    nodes = so.fd_farfield_nodes(pb.y, p.ymax)
+    jl = so.leading_mode(2, alp, v[:5 * p.ny], p.ny, omega_r=p.omega.real, candidates=near, nodes=nodes)
+    gpu_solver.set_profile(0, *ts_profile)
+    _, pt = _spoint(96, shift, 2)
+    runs = [gpu_solver.solve_leading([pt], _lib.MODE_PLAIN_F64, _lib.LEAD_KRYLOV, want_vectors=True),
+            gpu_solver.solve_leading([pt], _lib.MODE_MIXED, _lib.LEAD_INVIT, want_vectors=True,
+                                     shifts=[alp[jl] * (1.0 + 1e-3)], fallback_shifts=[shift])]
+    for out in runs:
+        _say(f"  info {out['info']} eval {out['evals'][0]} ff {out['farfield'][0]:.3e}")
+        assert out["info"][0] == _lib.INFO_OK
+        assert abs(out["evals"][0] - alp[jl]) <= 1e-9 * abs(alp[jl])
+        assert abs(out["evals"][0] - TS_ALPHA) < 1e-6
+        assert out["farfield"][0] == pytest.approx(so.farfield_ratio(v[:, jl], p.ny, nodes), rel=1e-5)
