@@ -443,10 +443,9 @@ int run_prep(stab_ctx *c, int npts, Batch &B, bool equilibrate) {
     }
     hipLaunchKernelGGL(k_cheb_sums, dim3(1), dim3(256), 0, c->stream, ny, D1, D1 + ny * ny, D1 + 2 * ny * ny);
   }
-  hipLaunchKernelGGL(k_nodes, dim3(npts), dim3(256), 0, c->stream, B.pts, B.ops, B.prof, B.coef, B.grid);
   hipLaunchKernelGGL(k_fill, dim3(64), dim3(256), 0, c->stream, B.norms, (size_t)4 * npts, 0.0);
+  hipLaunchKernelGGL(k_nodes, dim3(npts), dim3(256), 0, c->stream, B.pts, B.ops, B.prof, B.coef, B.grid, B.norms);
   hipLaunchKernelGGL(k_fill, dim3(256), dim3(256), 0, c->stream, B.scales, B.nscales, 1.0);
-  hipLaunchKernelGGL(k_norms, dim3(npts), dim3(256), 0, c->stream, B.pts, B.coef, B.ops, B.norms);
   if (equilibrate)   // row scales; the column scales stay 1
     hipLaunchKernelGGL(k_rowscale, dim3(npts), dim3(256), 0, c->stream, B.pts, B.coef, B.ops, B.scales);
   HIPCHECK(c, hipGetLastError());

@@ -212,3 +212,66 @@ __device__ void pencil_apply(const PointDev &P, const zd *__restrict__ coef, con
   }
   __syncthreads();
 }
+
+// Squared Frobenius norm of one row (i,p) of matrix `form` in closed form:
+// sum_j sum_q |bh D1_ij - vyy D2_ij + delta_ij dh|^2 expands into the row sums
+// S11, S22, S12 of the operators plus a correction at j = i.
+__device__ inline double row_norm2(const PointDev &P, const zd *node, const double *ops, int form, int i, int p,
+                                   int kind) {
+  const int ny = P.ny;
+  const double *D1 = ops + P.op_off, *D2 = D1 + (size_t)ny * ny, *dt2w = D2 + (size_t)ny * ny;
+  const double *sums = dt2w + ny;
+  const double s11 = sums[i], s22 = sums[ny + i], s12 = sums[2 * ny + i], sdt = sums[3 * ny];
+  if (kind == ROW_DIRICHLET) return abs2(dirichlet_diag(P, form));
+  const double d1 = D1[(size_t)i * ny + i], d2 = D2[(size_t)i * ny + i];
+  double s = 0.0;
+  for (int q = 0; q < 5; ++q) {
+    zd bh, dh; double vyy;
+    form_coef(P, node, form, p, q, P.shift, bh, vyy, dh);
+    zd off;
+    double body;
+    if (kind == ROW_GENERIC || (!walle_vyy_d1(P) && q < 4)) {
+      body = abs2(bh) * s11 + vyy * vyy * s22 - 2.0 * bh.x * vyy * s12;
+      off = bh * d1 - mk(vyy * d2, 0.0);
+    } else if (q < 4) {          // Chebyshev temporal wall energy row: (bh - vyy) D1
+      const zd b2 = bh - mk(vyy, 0.0);
+      body = abs2(b2) * s11;
+      off = b2 * d1;
+    } else {                     // column 4 of the wall energy row: -vyy Dt2(wall, :)
+      body = vyy * vyy * sdt;
+      off = mk(-vyy * dt2w[i], 0.0);
+    }
+    s += body - abs2(off) + abs2(off + dh);
+  }
+  return s;
+}
+
+// Per-point squared Frobenius norms of the pencil parts (the denominators of
+// the relative residuals):  temporal [0] = A, [1] = B (solver form);
+// spatial [0] = C0, [1] = C1, [2] = C2.  Called by every thread of a block
+// of 256, thread i for node i (live when i < ny), once the node's blocks
+// are in `coef`; thread 0 .. 2 write norms[0 .. 2].
+__device__ inline void point_norms(const PointDev &P, const zd *__restrict__ coef, const double *__restrict__ ops,
+                                   int i, double *__restrict__ norms) {
+  __shared__ double s_red[3][256];
+  double a[3] = {0.0, 0.0, 0.0};
+  if (i < P.ny) {
+    const zd *node = coef + P.coef_off + (size_t)i * NODE_STRIDE;
+    const int f0 = (P.itype == 1) ? FORM_AM : FORM_C0;
+    const int f1 = (P.itype == 1) ? FORM_BM : FORM_C1;
+    for (int p = 0; p < 5; ++p) {
+      const int kind = row_kind(P, 5 * i + p);
+      a[0] += row_norm2(P, node, ops, f0, i, p, kind);
+      a[1] += row_norm2(P, node, ops, f1, i, p, kind);
+      if (P.itype == 2) a[2] += row_norm2(P, node, ops, FORM_C2, i, p, kind);
+    }
+  }
+  for (int k = 0; k < 3; ++k) s_red[k][i] = a[k];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (i < w)
+      for (int k = 0; k < 3; ++k) s_red[k][i] += s_red[k][i + w];
+    __syncthreads();
+  }
+  if (i < 3) norms[i] = s_red[i][0];
+}

@@ -1,6 +1,6 @@
 // Preparation kernels: spline of the mean-flow profile, Chebyshev operators,
 // grid/mapping, interpolated mean flow and the per-node 5x5 coefficient blocks.
-#include "common.h"
+#include "pencil.h"
 #include "fd_stencils.h"
 
 // Second derivatives of the cubic spline through one profile column with the
@@ -80,7 +80,7 @@ __global__ void k_cheb_d2(int ny, const double *__restrict__ D1, double *__restr
   }
 }
 
-// row sums for the closed-form Frobenius norms of the pencil (k_norms):
+// row sums for the closed-form Frobenius norms of the pencil (point_norms):
 // S11_i = sum_j D1_ij^2, S22_i = sum_j D2_ij^2, S12_i = sum_j D1_ij D2_ij,
 // and Sdt = sum_j Dt2(wall, j)^2, after dt2w (ny entries)
 __global__ void k_cheb_sums(int ny, const double *__restrict__ D1, const double *__restrict__ D2,
@@ -530,11 +530,11 @@ __device__ inline double speval(const double *x, const double *v, const double *
          v[i] * dxp / del + v[i + 1] * dxm / del;
 }
 
-// One block per point, one thread per node.  Writes the grid (4 ny), vm (5 ny)
-// and the node coefficient slots.
+// One block per point, one thread per node.  Writes the grid (4 ny), vm (5 ny),
+// the node coefficient slots and the norms of the pencil parts (4 per point).
 __global__ __launch_bounds__(256) void k_nodes(const PointDev *__restrict__ pts, const double *__restrict__ ops,
                                                const double *__restrict__ prof, zd *__restrict__ coef,
-                                               double *__restrict__ grid) {
+                                               double *__restrict__ grid, double *__restrict__ norms) {
   const PointDev P = pts[blockIdx.x];
   const int ny = P.ny, i = threadIdx.x;
   const bool spatial = (P.itype == 2);
@@ -598,17 +598,15 @@ __global__ __launch_bounds__(256) void k_nodes(const PointDev *__restrict__ pts,
       s_g[k][i] = a[k];
     }
     __syncthreads();
-    if (i >= ny) return;
-    for (int k = 0; k < 5; ++k) {
+    for (int k = 0; k < 5 && i < ny; ++k) {
       double b = 0.0;
       for (int j = 0; j < ny; ++j) b += Dg[(size_t)j * ny + i] * s_g[k][j];
       g22[k] = b * (deta * deta) + a[k] * d2eta;
       g2[k] = a[k] * deta;
     }
   } else {
-    if (i >= ny) return;
     // mean derivatives by collocation, mapped to y (temporal.f90:148-179)
-    for (int k = 0; k < 5; ++k) {
+    for (int k = 0; k < 5 && i < ny; ++k) {
       double a = 0.0, b = 0.0;
       for (int j = 0; j < ny; ++j) {
         a += D1[(size_t)j * ny + i] * s_vm[k][j];
@@ -618,57 +616,61 @@ __global__ __launch_bounds__(256) void k_nodes(const PointDev *__restrict__ pts,
       g2[k] = a * deta;
     }
   }
-  NodeState s;
-  node_state(P, spatial, vm, g2, g22, s);
-  // fd_spatial.f90:899-907 subtracts the time term of the wall energy row
-  // (wallt = 2) with G of the node its assembly loops ended on, node ny-2 (or
-  // ny-1 when top = 1), not the wall's: the energy row of the spatial G is
-  // rho e_T (:574), so that term is -i omega rho(that node) in the T column.
-  const bool wall_g_quirk = spatial && P.fd && P.wallt == 2 && i == 0;
-  const double rho_q = s_vm[0][(P.top == 1) ? ny - 1 : ny - 2];
-  zd *out = coef + P.coef_off + (size_t)i * NODE_STRIDE;
-  const zd I = mk(0.0, 1.0);
-  for (int p = 0; p < 5; ++p) {
-    Row5 r;
-    if (spatial) spatial_row(P, s, p, r); else temporal_row(P, s, p, r);
-    for (int q = 0; q < 5; ++q) {
-      const int e = p * 5 + q;
-      zd bh, dh;
-      const double vyy = r.Vyy[q];
-      if (!spatial) {
-        const zd al = P.alpha, be = P.beta;
-        // Dh = D + im*alpha*A + im*beta*C + alpha**2*Vxx + alpha*beta*Vxz + beta**2*Vzz
-        dh = mk(r.D[q], 0.0);
-        dh = dh + (I * al) * r.A[q];
-        dh = dh + (I * be) * r.C[q];
-        dh = dh + (al * al) * r.Vxx[q];
-        dh = dh + (al * be) * r.Vxz[q];
-        dh = dh + (be * be) * r.Vzz[q];
-        // Bh = B - im*alpha*Vxy - im*beta*Vyz, then Bh*deta - Vyy*d2eta
-        bh = mk(r.B[q], 0.0);
-        bh = bh - (I * al) * r.Vxy[q];
-        bh = bh - (I * be) * r.Vyz[q];
-        bh = bh * deta - mk(vyy * d2eta, 0.0);
-      } else {
-        const zd be = P.beta, om = P.omega;
-        dh = mk(r.D[q], 0.0);
-        dh = dh + (I * be) * r.C[q];
-        dh = dh + (be * be) * r.Vzz[q];
-        const double gq = (wall_g_quirk && p == 4) ? (q == 4 ? rho_q : 0.0) : r.G[q];
-        dh = dh - (I * om) * gq;                         // time term (spatial.f90:808-809)
-        bh = mk(r.B[q], 0.0);
-        bh = bh - (I * be) * r.Vyz[q];
-        bh = bh * deta - mk(vyy * d2eta, 0.0);
-        out[SLOT_BH1 * 25 + e] = (mk(0.0, -1.0) * r.Vxy[q]) * deta;   // Bh = -im*Vxy, times deta
-        zd d1 = I * r.A[q];
-        d1 = d1 + be * r.Vxz[q];
-        out[SLOT_DH1 * 25 + e] = d1;
-        out[SLOT_VXX * 25 + e] = mk(r.Vxx[q], 0.0);
+  if (i < ny) {
+    NodeState s;
+    node_state(P, spatial, vm, g2, g22, s);
+    // fd_spatial.f90:899-907 subtracts the time term of the wall energy row
+    // (wallt = 2) with G of the node its assembly loops ended on, node ny-2 (or
+    // ny-1 when top = 1), not the wall's: the energy row of the spatial G is
+    // rho e_T (:574), so that term is -i omega rho(that node) in the T column.
+    const bool wall_g_quirk = spatial && P.fd && P.wallt == 2 && i == 0;
+    const double rho_q = s_vm[0][(P.top == 1) ? ny - 1 : ny - 2];
+    zd *out = coef + P.coef_off + (size_t)i * NODE_STRIDE;
+    const zd I = mk(0.0, 1.0);
+    for (int p = 0; p < 5; ++p) {
+      Row5 r;
+      if (spatial) spatial_row(P, s, p, r); else temporal_row(P, s, p, r);
+      for (int q = 0; q < 5; ++q) {
+        const int e = p * 5 + q;
+        zd bh, dh;
+        const double vyy = r.Vyy[q];
+        if (!spatial) {
+          const zd al = P.alpha, be = P.beta;
+          // Dh = D + im*alpha*A + im*beta*C + alpha**2*Vxx + alpha*beta*Vxz + beta**2*Vzz
+          dh = mk(r.D[q], 0.0);
+          dh = dh + (I * al) * r.A[q];
+          dh = dh + (I * be) * r.C[q];
+          dh = dh + (al * al) * r.Vxx[q];
+          dh = dh + (al * be) * r.Vxz[q];
+          dh = dh + (be * be) * r.Vzz[q];
+          // Bh = B - im*alpha*Vxy - im*beta*Vyz, then Bh*deta - Vyy*d2eta
+          bh = mk(r.B[q], 0.0);
+          bh = bh - (I * al) * r.Vxy[q];
+          bh = bh - (I * be) * r.Vyz[q];
+          bh = bh * deta - mk(vyy * d2eta, 0.0);
+        } else {
+          const zd be = P.beta, om = P.omega;
+          dh = mk(r.D[q], 0.0);
+          dh = dh + (I * be) * r.C[q];
+          dh = dh + (be * be) * r.Vzz[q];
+          const double gq = (wall_g_quirk && p == 4) ? (q == 4 ? rho_q : 0.0) : r.G[q];
+          dh = dh - (I * om) * gq;                         // time term (spatial.f90:808-809)
+          bh = mk(r.B[q], 0.0);
+          bh = bh - (I * be) * r.Vyz[q];
+          bh = bh * deta - mk(vyy * d2eta, 0.0);
+          out[SLOT_BH1 * 25 + e] = (mk(0.0, -1.0) * r.Vxy[q]) * deta;   // Bh = -im*Vxy, times deta
+          zd d1 = I * r.A[q];
+          d1 = d1 + be * r.Vxz[q];
+          out[SLOT_DH1 * 25 + e] = d1;
+          out[SLOT_VXX * 25 + e] = mk(r.Vxx[q], 0.0);
+        }
+        out[SLOT_BH * 25 + e] = bh;
+        out[SLOT_DH * 25 + e] = dh;
+        out[SLOT_VYY * 25 + e] = mk(vyy * (deta * deta), 0.0);
+        out[SLOT_G * 25 + e] = mk(r.G[q], 0.0);
       }
-      out[SLOT_BH * 25 + e] = bh;
-      out[SLOT_DH * 25 + e] = dh;
-      out[SLOT_VYY * 25 + e] = mk(vyy * (deta * deta), 0.0);
-      out[SLOT_G * 25 + e] = mk(r.G[q], 0.0);
     }
   }
+  // the norms of the pencil parts, from the blocks this thread just wrote
+  point_norms(P, coef, ops, i, norms + (size_t)blockIdx.x * 4);
 }
