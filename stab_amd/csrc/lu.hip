@@ -315,12 +315,72 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(R <= 2 && NT
           A[(size_t)(c0 + cc) * lda + k0 + i] = u;
         }
         __syncthreads();
-        // A22 strip -= L21 U12, one row slot at a time with G columns of
-        // independent accumulators; a wave skips the slots in which none of
-        // its 64 rows is below the panel and inside the matrix (the third
-        // slot of matrices of order <= 2 NT, for instance)
-        constexpr int G = (R >= 2) ? 2 : 4;
+        // A22 strip -= L21 U12.  FP32: on the matrix cores, as the trailing
+        // kernel does it (C tile = the update transposed: lane l holds strip
+        // columns 4 (l / 16) + i of row r0 + l % 16), one 16-row tile at a
+        // time; the B operand L21(r0 + l % 16, 4 s + l / 16) belongs to
+        // another lane of the same wave, so each wave stages the L21 rows of
+        // two of its tiles (32 rows x NB) in its quarter of s_u, which the
+        // strips do not otherwise use, and reads them back in operand
+        // order.  Tiles are whole: rows below the panel start at k0 + NB, a
+        // multiple of 16, and n is a multiple of 32.
         const int wrow = t - lane;
+        if constexpr (sizeof(T) == 4 && SW == 16 && NB == 16 && NT == 64 * NW && 32 * NB * NW <= SU) {
+          typedef typename Mfma<T>::v4 v4;
+          C *lst = s_u + (size_t)wv * 32 * NB;
+#pragma unroll
+          for (int j = 0; j < R; ++j) {
+            const int w0 = wrow + j * NT;
+            if (w0 >= n || w0 + 63 < k0 + NB) continue;           // wave-uniform
+#pragma unroll 1
+            for (int half = 0; half < 2; ++half) {
+              const int rb = w0 + 32 * half;                       // first row of the tile pair
+              if (rb >= n || rb + 31 < k0 + NB) continue;          // wave-uniform
+              if ((lane >> 5) == half) {
+#pragma unroll
+                for (int k = 0; k < NB; ++k) lst[(lane & 31) * NB + k] = Pn[j][k];
+              }
+              __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+              __builtin_amdgcn_wave_barrier();
+              __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll 1
+              for (int a = 0; a < 2; ++a) {
+                const int r0 = rb + 16 * a;
+                if (r0 >= n || r0 < k0 + NB) continue;               // wave-uniform
+                C *Ab = A + (size_t)(c0 + 4 * (lane >> 4)) * lda + r0 + (lane & 15);
+                v4 dr, di;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                  const C x = Ab[(size_t)i * lda];
+                  dr[i] = x.x;
+                  di[i] = x.y;
+                }
+#pragma unroll
+                for (int st = 0; st < NB / 4; ++st) {
+                  const C l = lst[(16 * a + (lane & 15)) * NB + 4 * st + (lane >> 4)];
+                  const C u = s_u12[(4 * st + (lane >> 4)) * SW + (lane & 15)];
+                  const T nur = -u.x, ui = u.y, nui = -u.y;
+                  dr = Mfma<T>::mac(nur, l.x, dr);
+                  dr = Mfma<T>::mac(ui, l.y, dr);
+                  di = Mfma<T>::mac(nui, l.x, di);
+                  di = Mfma<T>::mac(nur, l.y, di);
+                }
+#pragma unroll
+                for (int i = 0; i < 4; ++i) Ab[(size_t)i * lda] = mk<T>(dr[i], di[i]);
+              }
+              __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+              __builtin_amdgcn_wave_barrier();
+              __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            }
+          }
+          __syncthreads();
+          continue;
+        }
+        // VALU (FP64): one row slot at a time with G columns of independent
+        // accumulators; a wave skips the slots in which none of its 64 rows
+        // is below the panel and inside the matrix (the third slot of
+        // matrices of order <= 2 NT, for instance)
+        constexpr int G = (R >= 2) ? 2 : 4;
 #pragma unroll 1
         for (int cg = 0; cg < sw; cg += G) {
 #pragma unroll
