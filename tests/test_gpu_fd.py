@@ -255,3 +255,27 @@ def test_fds_leading_mode(gpu_solver, ts_profile):
         assert abs(out["evals"][0] - alp[jl]) <= 1e-9 * abs(alp[jl])
         assert abs(out["evals"][0] - TS_ALPHA) < 1e-6
         assert out["farfield"][0] == pytest.approx(so.farfield_ratio(v[:, jl], p.ny, nodes), rel=1e-5)
+
+
+@pytest.mark.parametrize("case", ["ny9", "ymax", "itype5"])
+def test_fd_arguments_are_checked(gpu_solver, ts_profile, case):
+    """The finite-difference closures need ny >= 10 and ogengrid needs
+    ymax > 2 yi (ogengrid.f90:21 divides by ymax - 2 yi); itypes past 4 are
+    not solves of this library.  Each is refused with STAB_E_ARG before any
+    kernel runs, and the context stays usable."""
+    from stab_amd import solver as sv
+    from stab_amd import _lib
+    _say(f"fd argument check: {case}")
+    gpu_solver.set_profile(0, *ts_profile)
+    _, pt = _point(64, TS_OMEGA, 2)
+    if case == "ny9":
+        pt.ny, msg = 9, "below 10"
+    elif case == "ymax":
+        pt.ymax, msg = 2.0 * pt.yi, "ymax > 2 yi"
+    else:
+        pt.itype, msg = 5, "not 1 to 4"
+    with pytest.raises(sv.StabError, match=msg):
+        gpu_solver.solve([pt], _lib.MODE_PLAIN_F64)
+    _, ok = _point(64, TS_OMEGA, 2)
+    res = gpu_solver.solve([ok], _lib.MODE_PLAIN_F64)
+    assert res.info[0] == _lib.INFO_OK
