@@ -81,8 +81,8 @@ __device__ inline int net_perm(const int *ipiv, int k0, int w, int *srow, int *s
 // the whole batch:
 //   k_lu_panel  one work-group per matrix whose order exceeds K0: factors the
 //               block column [K0, K0 + kb) (panels in registers, the in-block
-//               strips, the interchanges), stores the block's records, and
-//               forms X = inv(L11);
+//               strips, on the matrix cores in FP32, the interchanges),
+//               stores the block's records, and forms X = inv(L11);
 //   k_lu_trail  one work-group per 64-column strip (32 in FP64) right of the
 //               block, for every such matrix: the block's row permutation,
 //               U12 = X A12 and A22 -= L21 U12 on the matrix cores.

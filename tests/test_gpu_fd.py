@@ -279,3 +279,36 @@ def test_fd_arguments_are_checked(gpu_solver, ts_profile, case):
     _, ok = _point(64, TS_OMEGA, 2)
     res = gpu_solver.solve([ok], _lib.MODE_PLAIN_F64)
     assert res.info[0] == _lib.INFO_OK
+
+
+@pytest.mark.parametrize("itype", [3, 4])
+def test_fd_pencils_sutherland_oblique(gpu_solver, itype):
+    """Every tensor term with the finite-difference rows: Sutherland mu(T),
+    Pr = 0.72, M = 0.5, beta != 0 on a computed compressible Blasius profile
+    (temporal A0/B0, spatial C0/C1/C2 with wallt 2 and top 1), device against
+    the oracle at 1e-13 of each matrix's largest entry."""
+    from oracle import stab_oracle as so
+    from stab_amd import _lib
+    from stab_amd import meanflow as mf
+    from stab_amd import solver as sv
+    _say(f"fd pencils, Sutherland, oblique, itype {itype}")
+    te = 288.15
+    tab = mf.profile(mf.BlasiusSpec(Ma=0.5, Pr=0.72, law="sutherland", Te=te, eta_max=20.0, nrows=1200))
+    ym, vt = tab[:, 0], tab[:, 1:6].copy()
+    gpu_solver.set_profile(0, ym, vt)
+    head = f"1\n{te}\n0.5 1200 0.72\n24 {2.0 / 46.0!r} 50\n1\n1\n"
+    if itype == 3:
+        txt = head + "0 0 0 0\n3\n0.21 0.003\n0.12 0\n0\n"
+    else:
+        txt = head + "1 0 2 0\n4\n0.06 0.001\n0.12 0\n0\n0\n"
+    pt = sv.make_point(sv.read_input(txt), 0.1 + 0j, 2)
+    p = so.parse_input(txt)
+    if itype == 3:
+        refs = zip((_lib.MAT_A0, _lib.MAT_B0), so.fd_temporal_pencil(so.fd_setup(p, ym, vt)))
+    else:
+        refs = zip((_lib.MAT_C0, _lib.MAT_C1, _lib.MAT_C2), so.fd_spatial_pencil(so.fd_setup(p, ym, vt, spatial=True)))
+    for which, ref in refs:
+        got = gpu_solver.pencil(pt, which)
+        err = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+        _say(f"  matrix {which}: max rel err {err:.2e}")
+        assert err < 1e-13
