@@ -468,7 +468,7 @@ subroutine mspatial(ind)
   integer :: nym
   real(8), allocatable :: ym(:), vt(:,:)
   real(8) :: omin, omax, oinc, bmin, bmax, binc
-  integer :: ind1, ind2, ind_inc, dtype, nbody, io, ib, no, nb, iver, npts, nst, k, u, ios, slot
+  integer :: ind1, ind2, ind_inc, dtype, nbody, io, ib, no, nb, iver, npts, nst, k, u, ios, slot, itp
   real(8), allocatable :: xb(:), delta(:), xs(:)
   character(len=256) :: line
   type(stab_point), allocatable :: pts(:)
@@ -479,10 +479,7 @@ subroutine mspatial(ind)
   read (*, *) ind1, ind2, ind_inc
   read (*, *) dtype
   call read_guess_line()
-  if (dtype /= 0) then
-    write (*, *) 'dtype ', dtype, ': only the Chebyshev path (dtype 0) is built'
-    call exit(2)
-  end if
+  itp = merge(2, 4, dtype == 0)           ! mspatial.f90:85-89: spatial, or fd_spatial for dtype /= 0
   open (newunit=u, file='delta.dat', status='old', iostat=ios)
   if (ios /= 0) then
     write (*, *) 'ERROR: cannot open delta.dat'
@@ -542,7 +539,7 @@ subroutine mspatial(ind)
         call makename('eig', iver, names(iver))
         write (*, '(2(1x,i3,1x),6(1pe13.6,1x))') ind, iver, x, yi, real(omega), aimag(omega), real(beta), &
           aimag(beta)
-        pts(iver) = point_from_stuff(2, slot, omega / stab_phase_guess, max(1, min(stab_nev, STAB_MAX_NEV)))
+        pts(iver) = point_from_stuff(itp, slot, omega / stab_phase_guess, max(1, min(stab_nev, STAB_MAX_NEV)))
         inds(iver) = ind
         xs(iver) = x
       end do

@@ -194,6 +194,42 @@ EXAMPLE = os.path.join(ROOT, "stab_amd", "alpha_march")
 
 
 @pytest.mark.gpu
+def test_driver_multi_spatial_stations_fd(tmp_path):
+    """itype 8 with dtype 1 (mspatial.f90:85-89 calls fd_spatial): the
+    finite-difference spatial solve at each station's Yi = 2 delta on the
+    ogengrid grid (ymax 50), one batched call; eig.<k> holds the alpha
+    nearest omega / 0.35, equal to the oracle's companion eigenvalue."""
+    from oracle import stab_oracle as so
+    from stab_amd import solver as sv
+    print("fortran driver: multiple spatial, finite differences", flush=True)
+    for k in (1, 2):
+        shutil.copy(golden("TStest", "profile.0"), tmp_path / f"profile.{k}")
+    (tmp_path / "delta.dat").write_text("# x delta\n100.0 0.5\n200.0 0.6\n")
+    base = open(golden("thesis_TStest", "spatial.inp")).read().replace("96 1.0 0.0", "48 1.0 50.0")
+    lines = base.splitlines()
+    k = next(i for i, l in enumerate(lines) if l.strip().startswith("2") and "Spatial" in l)
+    sweep = lines[:k] + ["8\t\t! multiple spatial", "0.08 0.08 0.01", "0.0 0.0 0.0", "1 2 1", "1",
+                         "0.35 0.0 1"]
+    r = subprocess.run([BIN], input="\n".join(sweep) + "\n", capture_output=True, text=True, cwd=tmp_path,
+                       timeout=300)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    ym, vt = so.read_profile(golden("TStest", "profile.0"))
+    for st, (xs, dl) in enumerate(((100.0, 0.5), (200.0, 0.6)), start=1):
+        d = sv.read_evec_dat(str(tmp_path / f"eig.{st}"))
+        assert d["itype"] == 2 and d["ind"] == st and abs(d["yi"] - 2 * dl) < 1e-12
+        assert abs(d["y"][0]) == 0.0 and abs(d["y"][-1] - 50.0) < 1e-12     # ogengrid, wall first
+        ev = d["evals"][np.abs(d["evals"].real) < 1e299]
+        assert ev.size == 1
+        p = so.parse_input(base)
+        p.itype, p.omega, p.yi = 4, complex(0.08, 0.0), 2 * dl
+        alp, _ = so.spatial_solve_reference(*so.fd_spatial_pencil(so.fd_setup(p, ym, vt, spatial=True)), 0)
+        ref = alp[np.argmin(np.abs(alp - ev[0]))]
+        assert abs(ev[0] - ref) <= 1e-9 * abs(ref), (st, ev[0], ref)
+        assert abs(ev[0] - 0.08 / 0.35) < 0.05
+
+
+@pytest.mark.gpu
 def test_reference_style_caller_links_and_runs(tmp_path):
     """A caller written against the reference's interface only (module stuff,
     input, temporal(name, ind), makename) runs unchanged on this build: each
