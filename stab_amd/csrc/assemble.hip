@@ -42,6 +42,22 @@ __global__ __launch_bounds__(256) void k_rowscale(const PointDev *__restrict__ p
   }
 }
 
+// Streaming (non-temporal) store of one matrix entry, for matrices written
+// once and read back only after they have left the caches: the assembled
+// pencil (read by the LU, after the whole batch is written; L2 is left to the
+// operators every row reads) and the trailing update's A22 tiles.
+template <typename T> __device__ inline void store_stream(cx<T> *p, cx<T> v);
+template <> __device__ inline void store_stream<float>(zf *p, zf v) {
+  typedef float v2 __attribute__((ext_vector_type(2)));
+  v2 w = {v.x, v.y};
+  __builtin_nontemporal_store(w, (v2 *)p);
+}
+template <> __device__ inline void store_stream<double>(zd *p, zd v) {
+  typedef double v2 __attribute__((ext_vector_type(2)));
+  v2 w = {v.x, v.y};
+  __builtin_nontemporal_store(w, (v2 *)p);
+}
+
 // Write matrix `form` (scaled by dr, dc when `scaled`) column-major with
 // ld = npad.  grid: x = chunks of CW columns, y = point.
 template <typename T, int CW>
@@ -59,7 +75,7 @@ __global__ __launch_bounds__(256) void k_assemble(const PointDev *__restrict__ p
     if (c >= P.npad) break;
     zd v = pencil_entry(P, coef, ops, form, P.shift, r, c);
     if (scaled) v = v * (dr[r] * dc[c]);
-    M[(size_t)c * ld + r] = cvt<T>(v);
+    store_stream(M + (size_t)c * ld + r, cvt<T>(v));
   }
 }
 
@@ -89,7 +105,7 @@ __global__ __launch_bounds__(256) void k_assemble_rows(const PointDev *__restric
     for (int c = 0; c < P.npad; ++c) {
       zd v = (c == r) ? d : czero<double>();
       if (scaled) v = v * (rs * dc[c]);
-      M[(size_t)c * ld + r] = cvt<T>(v);
+      store_stream(M + (size_t)c * ld + r, cvt<T>(v));
     }
     return;
   }
@@ -115,10 +131,10 @@ __global__ __launch_bounds__(256) void k_assemble_rows(const PointDev *__restric
       if (i == j) v = v + dh[q];
       const int c = 5 * j + q;
       if (scaled) v = v * (rs * dc[c]);
-      M[(size_t)c * ld + r] = cvt<T>(v);
+      store_stream(M + (size_t)c * ld + r, cvt<T>(v));
     }
   }
-  for (int c = n; c < P.npad; ++c) M[(size_t)c * ld + r] = czero<T>();
+  for (int c = n; c < P.npad; ++c) store_stream(M + (size_t)c * ld + r, czero<T>());
 }
 
 template __global__ void k_assemble_rows<double>(const PointDev *, const zd *, const double *, const double *, zd *, int);

@@ -42,7 +42,11 @@ struct KSState {
 // whole factor once and is bound by how many bytes one work-group keeps in
 // flight, so every load is 16 bytes wide (two FP32 or one FP64 complex matrix
 // entries of consecutive rows) and each lane issues 16 of them before using
-// any (64 KB in flight per work-group).  Per block of 64 rows:
+// any (64 KB in flight per work-group).  The factor and block inverses are
+// read with streaming (non-temporal) loads: a point's next solve comes after
+// the other work-groups have streamed theirs, so they would only push the
+// operators D1, D2, which every pencil product rereads, out of L2.  Per block
+// of 64 rows:
 //   1. y_top <- inv(D) y_top: lane (i, q) sums columns 16q..16q+15 of row i
 //      of the column-major inverse (a wave reads 64 consecutive entries per
 //      load); the four partial sums meet in tmp (NT = 256 entries);
@@ -55,12 +59,12 @@ struct KSState {
 #define SOLVE_BS 64
 template <typename T> struct Row16;
 template <> struct Row16<float> {
-  typedef float4 V;
+  typedef float V __attribute__((ext_vector_type(4)));
   static constexpr int E = 2;
   __device__ static zd get(const V &v, int e) { return e == 0 ? mk((double)v.x, (double)v.y) : mk((double)v.z, (double)v.w); }
 };
 template <> struct Row16<double> {
-  typedef double2 V;
+  typedef double V __attribute__((ext_vector_type(2)));
   static constexpr int E = 1;
   __device__ static zd get(const V &v, int) { return mk(v.x, v.y); }
 };
@@ -77,7 +81,7 @@ __device__ inline void solve_diag(const cx<T> *__restrict__ inv, int k0, int bs,
   const cx<T> *col = inv + (size_t)(q * 16) * BS + i;
   cx<T> v[16];
 #pragma unroll
-  for (int j = 0; j < 16; ++j) v[j] = col[j * BS];
+  for (int j = 0; j < 16; ++j) v[j] = load_stream(col + j * BS);
   zd s[NR];
 #pragma unroll
   for (int r = 0; r < NR; ++r) s[r] = czero<double>();
@@ -124,7 +128,7 @@ __device__ inline void solve_update(const cx<T> *__restrict__ A, int lda, int k0
     const int row = rb + g * E, rl = min(row, r1 - E);
     V v[16];
 #pragma unroll
-    for (int j = 0; j < 16; ++j) v[j] = base[coff[j] + rl / E];
+    for (int j = 0; j < 16; ++j) v[j] = __builtin_nontemporal_load(base + coff[j] + rl / E);
     zd a[NR][E];
 #pragma unroll
     for (int r = 0; r < NR; ++r)

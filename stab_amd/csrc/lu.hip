@@ -544,6 +544,21 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(R <= 2 && NT
 // block.  Complex products as 2 x 2 real blocks on the matrix cores,
 // computed transposed (MFMA rows = matrix columns) so that each accumulator
 // register of 16 lanes covers 16 consecutive matrix rows.
+// Streaming load of one A22 entry: the trailing update reads each A22 tile
+// once per block step and writes it back with store_stream (the next read is
+// a block step later, after the batch's other tiles have passed through the
+// caches), so L2 is left to the L21 panel that every strip of a point reads.
+template <typename T> __device__ inline cx<T> load_stream(const cx<T> *p);
+template <> __device__ inline zf load_stream<float>(const zf *p) {
+  typedef float v2 __attribute__((ext_vector_type(2)));
+  const v2 w = __builtin_nontemporal_load((const v2 *)p);
+  return mk(w.x, w.y);
+}
+template <> __device__ inline zd load_stream<double>(const zd *p) {
+  typedef double v2 __attribute__((ext_vector_type(2)));
+  const v2 w = __builtin_nontemporal_load((const v2 *)p);
+  return mk(w.x, w.y);
+}
 template <typename T>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_lu_trail(const PointDev *__restrict__ pts, cx<T> *__restrict__ mats,
                                                   const int *__restrict__ pivs, const cx<T> *__restrict__ blks,
@@ -659,7 +674,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const bool ok = r0 < n && ct * 16 < gw;
-      C x = Ab[(ok ? (ct * 16 + Mfma<T>::row_step(i)) * lda + r0 : kend)];
+      C x = load_stream(Ab + (ok ? (ct * 16 + Mfma<T>::row_step(i)) * lda + r0 : kend));
       if (!ok) x = czero<T>();
       dr[ct][i] = x.x;
       di[ct][i] = x.y;
@@ -671,7 +686,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const bool ok = rn < n && ct * 16 < gw;
-        C x = Ab[(ok ? (ct * 16 + Mfma<T>::row_step(i)) * lda + rn : r0)];
+        C x = load_stream(Ab + (ok ? (ct * 16 + Mfma<T>::row_step(i)) * lda + rn : r0));
         if (!ok) x = czero<T>();
         nr[ct][i] = x.x;
         ni[ct][i] = x.y;
@@ -711,7 +726,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
       for (int i = 0; i < 4; ++i)
-        if (ct * 16 < gw) Ab[(ct * 16 + Mfma<T>::row_step(i)) * lda + r0] = mk<T>(dr[ct][i], di[ct][i]);
+        if (ct * 16 < gw) store_stream(Ab + (ct * 16 + Mfma<T>::row_step(i)) * lda + r0, mk<T>(dr[ct][i], di[ct][i]));
 #pragma unroll
     for (int ct = 0; ct < CT; ++ct) { dr[ct] = nr[ct]; di[ct] = ni[ct]; }
   }
@@ -727,8 +742,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 // q of column c sums the terms k = q (mod 4) of each substitution step and
 // keeps the entries x[k][c], k = q (mod 4), in registers; the step's sum is
 // reduced over the four lanes by shuffles, so the 64-step recurrence needs
-// no barrier.  Only the block itself lives in LDS, as stored (the arithmetic
-// converts to FP64).
+// no barrier.  The reciprocals of the 64 diagonal entries are formed once,
+// in LDS, so each step multiplies instead of dividing.  Besides them only the
+// block itself lives in LDS, as stored (the arithmetic converts to FP64).
 template <typename T, int BS>
 __global__ __launch_bounds__(4 * BS) void k_blkinv(const PointDev *__restrict__ pts, const cx<T> *__restrict__ mats,
                                                    cx<T> *__restrict__ blks) {
@@ -742,11 +758,16 @@ __global__ __launch_bounds__(4 * BS) void k_blkinv(const PointDev *__restrict__ 
   cx<T> *Ui = blks + P.blk_off + (size_t)(nblk + blockIdx.x) * BS * BS;
   constexpr int LD = BS + 1;       // padded rows: the column-wise staging writes hit distinct banks
   __shared__ cx<T> s_a[BS * LD];  // row-major copy of the block, as stored (FP32: 33 KB, so more groups per CU)
+  __shared__ zd s_rd[BS];          // 1 / U(i, i)
   for (int e = t; e < BS * BS; e += 4 * BS) {
     const int i = e % BS, j = e / BS;   // consecutive lanes read consecutive rows of column j
     const bool ok = i < bs && j < bs;
     const cx<T> v = A[(size_t)(k0 + (ok ? j : 0)) * n + k0 + (ok ? i : 0)];
     s_a[i * LD + j] = ok ? v : czero<T>();
+  }
+  if (t < BS) {
+    const int i = min(t, bs - 1);
+    s_rd[t] = cdiv(mk(1.0, 0.0), cvt<double>(A[(size_t)(k0 + i) * n + k0 + i]));
   }
   __syncthreads();
   const int c = t >> 2, q = t & 3;
@@ -768,7 +789,7 @@ __global__ __launch_bounds__(4 * BS) void k_blkinv(const PointDev *__restrict__ 
       s.x += quad_xor1(s.x); s.y += quad_xor1(s.y);
       s.x += quad_xor2(s.x); s.y += quad_xor2(s.y);
       const zd rhs = ((i == c) ? mk(1.0, 0.0) : czero<double>()) - s;
-      const zd v = (i > c || c >= bs) ? czero<double>() : cdiv(rhs, cvt<double>(s_a[i * LD + i]));
+      const zd v = (i > c || c >= bs) ? czero<double>() : rhs * s_rd[i];
       if (r == q) x[mi] = v;
     }
   }
