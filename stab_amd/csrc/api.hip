@@ -1362,7 +1362,15 @@ static int lead_invit(stab_ctx *c, int npts, const stab_point *pts, int mode, in
   // results, so both passes are issued together; the host applies the same
   // rule afterwards to read the answers.
   const size_t lds2 = lds + sizeof(zd) * (6 * (size_t)B.npadmax + 2 * 256);
-  const bool do2 = lds2 <= 160 * 1024;
+  // By default the second pass runs inside k_invit (fused): a point that
+  // needs it starts it as soon as its own first pass ends, beside the other
+  // points' first passes, instead of in a second launch whose few work-groups
+  // leave the GPU idle; it needs only 2 NT more LDS entries, as it reuses the
+  // first pass's vectors.  STAB_FUSE2=0 issues k_invit2 after k_invit.
+  const size_t lds_f = lds + sizeof(zd) * 2 * 256;
+  const bool fuse2 = env_int("STAB_FUSE2", 1) && lds_f <= 160 * 1024;
+  const bool do2 = fuse2 || lds2 <= 160 * 1024;
+  const size_t lds1 = fuse2 ? lds_f : lds;
   // The whole pass (operators, node coefficients, assembly, LU, block
   // inverses, inverse iteration, second pass) depends on the host only through
   // the point descriptors, which build_batch has copied; it is issued as one
@@ -1393,19 +1401,22 @@ static int lead_invit(stab_ctx *c, int npts, const stab_point *pts, int mode, in
     if ((rc2 = f32 ? launch_lu<float, 16>(c, npts, B, false, !use_graph) : launch_lu<double, 8>(c, npts, B, false, !use_graph)) != STAB_OK)
       return rc2;
     if ((rc2 = mark(3)) != STAB_OK) return rc2;
+    zd *ev2 = B.eval + npts;
+    double *rs2 = B.resid + npts, *ff2 = B.ff + npts;    // 2 npts entries each (KS_NEVMAX npts allocated)
+    const int want = nearest ? 1 : 0;
     if (f32)
-      hipLaunchKernelGGL((k_invit<float, 16, 256>), dim3(npts), dim3(256), lds, c->stream, B.pts, (const zf *)B.mats,
+      hipLaunchKernelGGL((k_invit<float, 16, 256>), dim3(npts), dim3(256), lds1, c->stream, B.pts, (const zf *)B.mats,
                          B.pivs, (const zf *)B.blks, B.scales, B.coef, B.ops, B.norms, maxit, tol, B.eval, B.evec,
-                         B.resid, B.ff, B.iters, B.flag, B.npadmax, B.nymax, B.order);
+                         B.resid, B.ff, B.iters, B.flag, B.npadmax, B.nymax, B.order, fuse2 ? 2 : (do2 ? 1 : 0), want,
+                         (const int *)B.info, ev2, rs2, ff2);
     else
-      hipLaunchKernelGGL((k_invit<double, 16, 256>), dim3(npts), dim3(256), lds, c->stream, B.pts, (const zd *)B.mats,
+      hipLaunchKernelGGL((k_invit<double, 16, 256>), dim3(npts), dim3(256), lds1, c->stream, B.pts, (const zd *)B.mats,
                          B.pivs, (const zd *)B.blks, B.scales, B.coef, B.ops, B.norms, maxit, tol, B.eval, B.evec,
-                         B.resid, B.ff, B.iters, B.flag, B.npadmax, B.nymax, B.order);
+                         B.resid, B.ff, B.iters, B.flag, B.npadmax, B.nymax, B.order, fuse2 ? 2 : (do2 ? 1 : 0), want,
+                         (const int *)B.info, ev2, rs2, ff2);
     HIPCHECK(c, hipGetLastError());
     if ((rc2 = mark(4)) != STAB_OK) return rc2;
-    if (do2) {
-      zd *ev2 = B.eval + npts;
-      double *rs2 = B.resid + npts, *ff2 = B.ff + npts;    // 2 npts entries each (KS_NEVMAX npts allocated)
+    if (do2 && !fuse2) {
       if (f32)
         hipLaunchKernelGGL((k_invit2<float, 16, 256>), dim3(npts), dim3(256), lds2, c->stream, B.pts, nullptr,
                            (const zf *)B.mats, B.pivs, (const zf *)B.blks, B.scales, B.coef, B.ops, B.norms, 2 * maxit,
@@ -1422,7 +1433,7 @@ static int lead_invit(stab_ctx *c, int npts, const stab_point *pts, int mode, in
   };
   if (use_graph) {
     std::vector<long long> sig = {npts, f32, maxit, (long long)lds, B.npadmax, B.nymax, B.nvecmax,
-                                  (long long)B.nscales, (long long)lds2, do2, nearest};
+                                  (long long)B.nscales, (long long)lds2, do2, nearest, fuse2};
     for (int key : B.nys) { sig.push_back(key); sig.push_back(B.op_off[key]); }
     for (const void *q : {(const void *)B.pts, (const void *)B.prof, (const void *)B.ops, (const void *)B.coef,
                           (const void *)B.grid, (const void *)B.mats, (const void *)B.pivs, (const void *)B.blks,

@@ -964,177 +964,6 @@ __global__ __launch_bounds__(NT) void k_solve_one(const PointDev *__restrict__ p
   for (int r = threadIdx.x; r < P.n; r += NT) x[r] = L.y[r];
 }
 
-// ---------------------------------------------------------------------------
-// Residual inverse iteration (A. Neumaier, "Residual inverse iteration for the
-// nonlinear eigenvalue problem", SIAM J. Numer. Anal. 22 (1985) 914-923) for
-// the eigenpair nearest the factored shift sigma:
-//     x <- x - L(sigma)^-1 L(lambda) x,   lambda: root of x^H L(lambda) x = 0
-// nearest the previous value, with L(l) = A - l B (temporal solver pencil) or
-// C0 + l C1 + l^2 C2 (spatial, on phi directly).  The correction
-// L(sigma)^-1 r shrinks with the residual r, so FP32 factors limit only the
-// contraction per step (|lambda - sigma| / |lambda_2 - sigma| plus about
-// kappa u32), not the attainable accuracy, which the FP64 residual sets.
-// The first step is one plain inverse-iteration step x = L(sigma)^-1 L'(sigma) r
-// from a hashed start vector.  One work-group per point.  Stops when the FP64
-// relative residual is below tol, when it has not halved for three steps while
-// below 1e-12 (the rounding floor), or after maxit corrections.  Outputs per point: lambda, the
-// vector normalised by its first entry of largest modulus (z = [phi;
-// lambda phi] for spatial points, at out_off), the residual, the farfield
-// ratio, the number of solves and a convergence flag (0, or 2 when the
-// residual stayed above 1e-14).
-// ---------------------------------------------------------------------------
-template <typename T, int NB, int NT>
-__global__ __launch_bounds__(NT) void k_invit(const PointDev *__restrict__ pts, const cx<T> *__restrict__ mats,
-                                              const int *__restrict__ pivs, const cx<T> *__restrict__ blks,
-                                              const double *__restrict__ scales, const zd *__restrict__ coef,
-                                              const double *__restrict__ ops, const double *__restrict__ norms,
-                                              int maxit, double tol, zd *__restrict__ eval, zd *__restrict__ evec,
-                                              double *__restrict__ resid, double *__restrict__ ffo,
-                                              int *__restrict__ iters, int *__restrict__ flag, int npadmax,
-                                              int nymax, const int *__restrict__ order) {
-  extern __shared__ zd lds[];
-  const int pi = order ? order[blockIdx.x] : blockIdx.x;   // largest points first
-  const PointDev &P = pts[pi];
-  OpLds L(lds, P, npadmax, nymax, 1, npadmax);
-  const int t = threadIdx.x, n = P.n;
-  const cx<T> *A = mats + P.mat_off;
-  const int *ipiv = pivs + P.piv_off;
-  const cx<T> *Lb = blks + P.blk_off, *Ub = nullptr;
-  const double *dr = scales + P.scale_off, *dc = dr + P.npad;
-  const bool temporal = (P.itype == 1);
-  const double nA = sqrt(norms[pi * 4 + 0]), nB = sqrt(norms[pi * 4 + 1]),
-               nC = sqrt(norms[pi * 4 + 2]);
-  const zd sg = P.shift;
-  zd *x = L.z;
-  __shared__ double s_mx[NT];
-  __shared__ int s_ix[NT];
-
-  // ---- start: x = L(sigma)^-1 L'(sigma) r
-  const unsigned seed = point_seed(P);
-  for (int r = t; r < n; r += NT) x[r] = mk(hash01(seed, 2 * r), hash01(seed, 2 * r + 1));
-  __syncthreads();
-  if (temporal) {
-    pencil_apply<NT>(P, coef, ops, FORM_BM, sg, x, L.rhs, L.X1, L.X2, L.red);
-  } else {
-    pencil_apply<NT>(P, coef, ops, FORM_C1, sg, x, L.rhs, L.X1, L.X2, L.red);
-    pencil_apply<NT>(P, coef, ops, FORM_C2, sg, x, L.res, L.X1, L.X2, L.red);
-    for (int r = t; r < n; r += NT) L.rhs[r] = L.rhs[r] + (sg * 2.0) * L.res[r];
-    __syncthreads();
-  }
-  shifted_solve<T, NB, NT>(P, A, ipiv, Lb, Ub, dr, dc, coef, ops, 0, L);
-  for (int r = t; r < n; r += NT) x[r] = L.y[r];
-  __syncthreads();
-  int nsolve = 1;
-
-  zd lam = sg;
-  double rel = 1e300, best = 1e300;
-  int stall = 0;
-  for (int it = 0;; ++it) {
-    // unit 2-norm
-    {
-      const double nx = sqrt(block_norm2<NT>(x, n, L.red));
-      const double inv = (nx > 0.0) ? 1.0 / nx : 1.0;
-      for (int r = t; r < n; r += NT) x[r] = x[r] * inv;
-      __syncthreads();
-    }
-    // pencil products and the new lambda
-    zd d3[3] = {czero<double>(), czero<double>(), czero<double>()};
-    if (temporal) {
-      pencil_apply<NT>(P, coef, ops, FORM_AM, sg, x, L.y, L.X1, L.X2, L.red);
-      pencil_apply<NT>(P, coef, ops, FORM_BM, sg, x, L.rhs, L.X1, L.X2, L.red);
-      for (int r = t; r < n; r += NT) {
-        d3[0] = cfmac(d3[0], x[r], L.y[r]);
-        d3[1] = cfmac(d3[1], x[r], L.rhs[r]);
-      }
-      block_reduce<NT, 3>(d3, L.red);
-      if (cabs(d3[1]) > 0.0) lam = cdiv(d3[0], d3[1]);
-    } else {
-      pencil_apply<NT>(P, coef, ops, FORM_C0, sg, x, L.y, L.X1, L.X2, L.red);
-      pencil_apply<NT>(P, coef, ops, FORM_C1, sg, x, L.rhs, L.X1, L.X2, L.red);
-      pencil_apply<NT>(P, coef, ops, FORM_C2, sg, x, L.res, L.X1, L.X2, L.red);
-      for (int r = t; r < n; r += NT) {
-        d3[0] = cfmac(d3[0], x[r], L.y[r]);
-        d3[1] = cfmac(d3[1], x[r], L.rhs[r]);
-        d3[2] = cfmac(d3[2], x[r], L.res[r]);
-      }
-      block_reduce<NT, 3>(d3, L.red);
-      if (cabs(d3[2]) > 0.0) {
-        const zd bb = cdiv(d3[1], d3[2] * 2.0), cc = cdiv(d3[0], d3[2]);
-        const zd disc = bb * bb - cc;
-        const double rr = cabs(disc);
-        const double sr = sqrt(0.5 * (rr + fabs(disc.x)));
-        zd sq;
-        if (sr == 0.0) sq = czero<double>();
-        else if (disc.x >= 0) sq = mk(sr, disc.y / (2.0 * sr));
-        else sq = mk(fabs(disc.y) / (2.0 * sr), disc.y >= 0 ? sr : -sr);
-        const zd r1 = czero<double>() - bb + sq, r2 = czero<double>() - bb - sq;
-        lam = (cabs(r1 - lam) < cabs(r2 - lam)) ? r1 : r2;
-      } else if (cabs(d3[1]) > 0.0) {
-        lam = czero<double>() - cdiv(d3[0], d3[1]);
-      }
-    }
-    // residual r = L(lambda) x into rhs
-    zd part[1] = {czero<double>()};
-    if (temporal) {
-      for (int r = t; r < n; r += NT) {
-        const zd v = L.y[r] - lam * L.rhs[r];
-        L.rhs[r] = v;
-        part[0].x += abs2(v);
-      }
-    } else {
-      const zd l2 = lam * lam;
-      for (int r = t; r < n; r += NT) {
-        const zd v = L.y[r] + lam * L.rhs[r] + l2 * L.res[r];
-        L.rhs[r] = v;
-        part[0].x += abs2(v);
-      }
-    }
-    block_reduce<NT, 1>(part, L.red);
-    const double al = cabs(lam);
-    const double den = temporal ? (nA + al * nB) : (nA + al * nB + al * al * nC);
-    rel = sqrt(part[0].x) / den;             // ||x|| = 1
-    // the stall count only runs near the rounding floor: far from it the
-    // residual can grow for a step or two while the vector turns
-    if (rel < 0.5 * best) { best = rel; stall = 0; } else if (rel < 1e-12) { ++stall; }
-    if (rel < tol || it >= maxit || stall >= 3) break;
-    // correction
-    shifted_solve<T, NB, NT>(P, A, ipiv, Lb, Ub, dr, dc, coef, ops, 0, L);
-    ++nsolve;
-    for (int r = t; r < n; r += NT) x[r] = x[r] - L.y[r];
-    __syncthreads();
-  }
-  // ---- getevec normalisation: first entry of largest modulus set to 1
-  double bm = -1.0;
-  int bi = 0x7fffffff;
-  for (int r = t; r < n; r += NT) { const double a = abs2(x[r]); if (a > bm) { bm = a; bi = r; } }
-  s_mx[t] = bm; s_ix[t] = bi;
-  __syncthreads();
-  if (t == 0) {
-    for (int k = 1; k < NT; ++k)
-      if (s_mx[k] > bm || (s_mx[k] == bm && s_ix[k] < bi)) { bm = s_mx[k]; bi = s_ix[k]; }
-    s_ix[0] = clamp_index(bi, n);     // all-NaN vector: no entry compares greater
-  }
-  __syncthreads();
-  const zd scale = x[s_ix[0]];
-  __syncthreads();
-  if (scale_ok(scale))
-    for (int r = t; r < n; r += NT) x[r] = cdiv(x[r], scale);
-  __syncthreads();
-  const double ff = farfield_ratio<NT>(P, x, L.red);
-  zd *out = evec + P.out_off;
-  for (int r = t; r < n; r += NT) {
-    out[r] = x[r];
-    if (!temporal) out[n + r] = lam * x[r];
-  }
-  if (t == 0) {
-    eval[pi] = lam;
-    resid[pi] = rel;
-    ffo[pi] = ff;
-    iters[pi] = nsolve;
-    flag[pi] = (rel < 1e-14) ? 0 : 2;
-  }
-}
-
 // Principal square root of a complex number.
 __device__ inline zd zsqrt(zd z) {
   const double r = cabs(z);
@@ -1163,11 +992,10 @@ __device__ inline zd zsqrt(zd z) {
 // is iterated until it has converged and the other pair has converged too, or
 // has a residual below 1e-10 and can no longer outrank it (lead2_competes,
 // the test the host applies to the results); at most maxit steps.
-// grid: one work-group per listed point p = sub[blockIdx.x].  Outputs, k = 0, 1:
-// eval2[2 p + k], resid2[2 p + k], ff2[2 p + k] and the Ritz vector
-// (getevec normalisation) in column k of p's block; iters[p] += solves.
-// LDS: the full operator layout followed by six npadmax vectors and 2 NT
-// entries.
+// Outputs, k = 0, 1: eval2[2 p + k], resid2[2 p + k], ff2[2 p + k] and the
+// Ritz vector (getevec normalisation) in column k of p's block;
+// iters[p] += solves.  It runs inside k_invit after a point's first pass
+// (fused) or as k_invit2 after the whole first pass.
 // ---------------------------------------------------------------------------
 // The discrete-mode test of the leading-mode selection for temporal points:
 // farfield ratio below STAB_FARFIELD_MAX and phase speed off the continuum.
@@ -1190,35 +1018,34 @@ __host__ __device__ inline bool lead2_competes(int want, double alpha_r, zd sigm
   return maybe_discrete && th_o.y > th_w.y - err;
 }
 
+// The second pass for point p (P = its descriptor), x1 holding the first
+// pass's vector (getevec normalisation) on entry.  x1, x2, a1, a2 and tmp2
+// (2 NT entries) are LDS; b1 and b2, read only by the lane that wrote them,
+// may be global.  L supplies the solve's scratch (y, rhs, tmp, w2), the
+// products' (X1, X2, red) and nothing else, so x1 .. a2 may alias L.z, L.res,
+// L.y and L.rhs as long as a1, a2 are not L.rhs, L.y before the first step's
+// solve is done (they are written only inside the loop).
+// Whether the second pass takes a point after the first (maxit1 steps):
+// a temporal point whose LU succeeded and which the first pass left
+// unconverged, not discrete (unless want == 1), or converged only after 3/4
+// of its steps.  The host applies the same rule to the same data.
+__host__ __device__ inline bool lead2_needed(int info, int itype, int flag, int want, double alpha_r, zd ev,
+                                             double ff, int iters, int maxit1) {
+  if (info != 0 || itype != 1) return false;
+  const bool ok1 = flag == 0 && (want == 1 || lead2_discrete(alpha_r, ev, ff));
+  return !(ok1 && 4 * iters < 3 * (maxit1 + 1));
+}
+
 template <typename T, int NB, int NT>
-__global__ __launch_bounds__(NT) void k_invit2(const PointDev *__restrict__ pts, const int *__restrict__ sub,
-                                               const cx<T> *__restrict__ mats, const int *__restrict__ pivs,
-                                               const cx<T> *__restrict__ blks, const double *__restrict__ scales,
-                                               const zd *__restrict__ coef, const double *__restrict__ ops,
-                                               const double *__restrict__ norms, int maxit, double tol, int want,
-                                               zd *__restrict__ eval2, zd *__restrict__ evec,
-                                               double *__restrict__ resid2, double *__restrict__ ff2,
-                                               int *__restrict__ iters, int npadmax, int nymax,
-                                               const int *__restrict__ order, const int *__restrict__ info,
-                                               const int *__restrict__ flag, const zd *__restrict__ eval1,
-                                               const double *__restrict__ ff1, int maxit1) {
-  extern __shared__ zd lds[];
-  // sub lists the points; without it work-group g takes point order[g] and
-  // runs only when the first pass (maxit1 steps) left that temporal point
-  // unconverged, not discrete (unless want == 1), or converged only after
-  // 3/4 of its steps -- the host applies the same rule to the same data
-  const int p = sub ? sub[blockIdx.x] : order[blockIdx.x];
-  if (!sub) {
-    const PointDev &Q = pts[p];
-    if (info[p] != 0 || Q.itype != 1) return;
-    const bool ok1 = flag[p] == 0 && (want == 1 || lead2_discrete(Q.alpha.x, eval1[p], ff1[p]));
-    if (ok1 && 4 * iters[p] < 3 * (maxit1 + 1)) return;
-  }
-  const PointDev &P = pts[p];
-  OpLds L(lds, P, npadmax, nymax, 1, npadmax);
-  zd *x1 = L.X2 + 5 * nymax;
-  zd *x2 = x1 + npadmax, *a1 = x2 + npadmax, *b1 = a1 + npadmax, *a2 = b1 + npadmax, *b2 = a2 + npadmax;
-  zd *tmp2 = b2 + npadmax;                     // 2 NT entries for the two-column solve
+__device__ __forceinline__ void invit2_point(const PointDev &P, int p, const cx<T> *__restrict__ mats,
+                                             const int *__restrict__ pivs, const cx<T> *__restrict__ blks,
+                                             const double *__restrict__ scales, const zd *__restrict__ coef,
+                                             const double *__restrict__ ops, const double *__restrict__ norms,
+                                             int maxit, double tol, int want, zd *__restrict__ eval2,
+                                             zd *__restrict__ evec, double *__restrict__ resid2,
+                                             double *__restrict__ ff2, int *__restrict__ iters, OpLds &L, zd *x1,
+                                             zd *x2, zd *a1, zd *b1, zd *a2, zd *b2, zd *tmp2, double *s_mx,
+                                             int *s_ix) {
   const int t = threadIdx.x, n = P.n, nv = P.nvec;
   const cx<T> *A = mats + P.mat_off;
   const int *ipiv = pivs + P.piv_off;
@@ -1227,11 +1054,6 @@ __global__ __launch_bounds__(NT) void k_invit2(const PointDev *__restrict__ pts,
   const double nA = sqrt(norms[p * 4 + 0]), nB = sqrt(norms[p * 4 + 1]);
   const zd sg = P.shift;
   zd *out = evec + P.out_off;
-  __shared__ double s_mx[NT];
-  __shared__ int s_ix[NT];
-
-  for (int r = t; r < n; r += NT) x1[r] = out[r];
-  __syncthreads();
   pencil_apply<NT>(P, coef, ops, FORM_BM, sg, x1, L.rhs, L.X1, L.X2, L.red);
   shifted_solve<T, NB, NT>(P, A, ipiv, Lb, Ub, dr, dc, coef, ops, 0, L);
   for (int r = t; r < n; r += NT) x2[r] = L.y[r];
@@ -1393,6 +1215,250 @@ __global__ __launch_bounds__(NT) void k_invit2(const PointDev *__restrict__ pts,
     }
   }
   if (t == 0) iters[p] += nsolve;
+}
+
+
+// ---------------------------------------------------------------------------
+// Residual inverse iteration (A. Neumaier, "Residual inverse iteration for the
+// nonlinear eigenvalue problem", SIAM J. Numer. Anal. 22 (1985) 914-923) for
+// the eigenpair nearest the factored shift sigma:
+//     x <- x - L(sigma)^-1 L(lambda) x,   lambda: root of x^H L(lambda) x = 0
+// nearest the previous value, with L(l) = A - l B (temporal solver pencil) or
+// C0 + l C1 + l^2 C2 (spatial, on phi directly).  The correction
+// L(sigma)^-1 r shrinks with the residual r, so FP32 factors limit only the
+// contraction per step (|lambda - sigma| / |lambda_2 - sigma| plus about
+// kappa u32), not the attainable accuracy, which the FP64 residual sets.
+// The first step is one plain inverse-iteration step x = L(sigma)^-1 L'(sigma) r
+// from a hashed start vector.  One work-group per point.  Stops when the FP64
+// relative residual is below tol, when it has not halved for three steps while
+// below 1e-12 (the rounding floor), after maxit corrections, or (temporal
+// points with a second pass to follow, from the sixth correction on, above
+// 1e-12) when its rate over the last three steps predicts more than
+// 3/4 (maxit + 1) solves in all.  Outputs per point: lambda, the
+// vector normalised by its first entry of largest modulus (z = [phi;
+// lambda phi] for spatial points, at out_off), the residual, the farfield
+// ratio, the number of solves and a convergence flag (0, or 2 when the
+// residual stayed above 1e-14).  pass2: 0 when no second pass follows, 1
+// when k_invit2 follows, 2 when the work-group itself then runs the second
+// pass (invit2_point) for a point that needs it (the launch then holds 2 NT
+// more LDS entries).
+// ---------------------------------------------------------------------------
+template <typename T, int NB, int NT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_invit(const PointDev *__restrict__ pts, const cx<T> *__restrict__ mats,
+                                              const int *__restrict__ pivs, const cx<T> *__restrict__ blks,
+                                              const double *__restrict__ scales, const zd *__restrict__ coef,
+                                              const double *__restrict__ ops, const double *__restrict__ norms,
+                                              int maxit, double tol, zd *__restrict__ eval, zd *__restrict__ evec,
+                                              double *__restrict__ resid, double *__restrict__ ffo,
+                                              int *__restrict__ iters, int *__restrict__ flag, int npadmax,
+                                              int nymax, const int *__restrict__ order, int pass2, int want,
+                                              const int *__restrict__ info, zd *__restrict__ eval2,
+                                              double *__restrict__ resid2, double *__restrict__ ff2) {
+  extern __shared__ zd lds[];
+  const int pi = order ? order[blockIdx.x] : blockIdx.x;   // largest points first
+  const PointDev &P = pts[pi];
+  OpLds L(lds, P, npadmax, nymax, 1, npadmax);
+  const int t = threadIdx.x, n = P.n;
+  const cx<T> *A = mats + P.mat_off;
+  const int *ipiv = pivs + P.piv_off;
+  const cx<T> *Lb = blks + P.blk_off, *Ub = nullptr;
+  const double *dr = scales + P.scale_off, *dc = dr + P.npad;
+  const bool temporal = (P.itype == 1);
+  const double nA = sqrt(norms[pi * 4 + 0]), nB = sqrt(norms[pi * 4 + 1]),
+               nC = sqrt(norms[pi * 4 + 2]);
+  const zd sg = P.shift;
+  zd *x = L.z;
+  __shared__ double s_mx[NT];
+  __shared__ int s_ix[NT];
+
+  // ---- start: x = L(sigma)^-1 L'(sigma) r
+  const unsigned seed = point_seed(P);
+  for (int r = t; r < n; r += NT) x[r] = mk(hash01(seed, 2 * r), hash01(seed, 2 * r + 1));
+  __syncthreads();
+  if (temporal) {
+    pencil_apply<NT>(P, coef, ops, FORM_BM, sg, x, L.rhs, L.X1, L.X2, L.red);
+  } else {
+    pencil_apply<NT>(P, coef, ops, FORM_C1, sg, x, L.rhs, L.X1, L.X2, L.red);
+    pencil_apply<NT>(P, coef, ops, FORM_C2, sg, x, L.res, L.X1, L.X2, L.red);
+    for (int r = t; r < n; r += NT) L.rhs[r] = L.rhs[r] + (sg * 2.0) * L.res[r];
+    __syncthreads();
+  }
+  shifted_solve<T, NB, NT>(P, A, ipiv, Lb, Ub, dr, dc, coef, ops, 0, L);
+  for (int r = t; r < n; r += NT) x[r] = L.y[r];
+  __syncthreads();
+  int nsolve = 1;
+
+  zd lam = sg;
+  double rel = 1e300, best = 1e300;
+  double back[3] = {1e300, 1e300, 1e300};   // rel one, two and three steps back
+  int stall = 0;
+  for (int it = 0;; ++it) {
+    // unit 2-norm
+    {
+      const double nx = sqrt(block_norm2<NT>(x, n, L.red));
+      const double inv = (nx > 0.0) ? 1.0 / nx : 1.0;
+      for (int r = t; r < n; r += NT) x[r] = x[r] * inv;
+      __syncthreads();
+    }
+    // pencil products and the new lambda
+    zd d3[3] = {czero<double>(), czero<double>(), czero<double>()};
+    if (temporal) {
+      pencil_apply<NT>(P, coef, ops, FORM_AM, sg, x, L.y, L.X1, L.X2, L.red);
+      pencil_apply<NT>(P, coef, ops, FORM_BM, sg, x, L.rhs, L.X1, L.X2, L.red);
+      for (int r = t; r < n; r += NT) {
+        d3[0] = cfmac(d3[0], x[r], L.y[r]);
+        d3[1] = cfmac(d3[1], x[r], L.rhs[r]);
+      }
+      block_reduce<NT, 3>(d3, L.red);
+      if (cabs(d3[1]) > 0.0) lam = cdiv(d3[0], d3[1]);
+    } else {
+      pencil_apply<NT>(P, coef, ops, FORM_C0, sg, x, L.y, L.X1, L.X2, L.red);
+      pencil_apply<NT>(P, coef, ops, FORM_C1, sg, x, L.rhs, L.X1, L.X2, L.red);
+      pencil_apply<NT>(P, coef, ops, FORM_C2, sg, x, L.res, L.X1, L.X2, L.red);
+      for (int r = t; r < n; r += NT) {
+        d3[0] = cfmac(d3[0], x[r], L.y[r]);
+        d3[1] = cfmac(d3[1], x[r], L.rhs[r]);
+        d3[2] = cfmac(d3[2], x[r], L.res[r]);
+      }
+      block_reduce<NT, 3>(d3, L.red);
+      if (cabs(d3[2]) > 0.0) {
+        const zd bb = cdiv(d3[1], d3[2] * 2.0), cc = cdiv(d3[0], d3[2]);
+        const zd disc = bb * bb - cc;
+        const double rr = cabs(disc);
+        const double sr = sqrt(0.5 * (rr + fabs(disc.x)));
+        zd sq;
+        if (sr == 0.0) sq = czero<double>();
+        else if (disc.x >= 0) sq = mk(sr, disc.y / (2.0 * sr));
+        else sq = mk(fabs(disc.y) / (2.0 * sr), disc.y >= 0 ? sr : -sr);
+        const zd r1 = czero<double>() - bb + sq, r2 = czero<double>() - bb - sq;
+        lam = (cabs(r1 - lam) < cabs(r2 - lam)) ? r1 : r2;
+      } else if (cabs(d3[1]) > 0.0) {
+        lam = czero<double>() - cdiv(d3[0], d3[1]);
+      }
+    }
+    // residual r = L(lambda) x into rhs
+    zd part[1] = {czero<double>()};
+    if (temporal) {
+      for (int r = t; r < n; r += NT) {
+        const zd v = L.y[r] - lam * L.rhs[r];
+        L.rhs[r] = v;
+        part[0].x += abs2(v);
+      }
+    } else {
+      const zd l2 = lam * lam;
+      for (int r = t; r < n; r += NT) {
+        const zd v = L.y[r] + lam * L.rhs[r] + l2 * L.res[r];
+        L.rhs[r] = v;
+        part[0].x += abs2(v);
+      }
+    }
+    block_reduce<NT, 1>(part, L.red);
+    const double al = cabs(lam);
+    const double den = temporal ? (nA + al * nB) : (nA + al * nB + al * al * nC);
+    rel = sqrt(part[0].x) / den;             // ||x|| = 1
+    // the stall count only runs near the rounding floor: far from it the
+    // residual can grow for a step or two while the vector turns
+    if (rel < 0.5 * best) { best = rel; stall = 0; } else if (rel < 1e-12) { ++stall; }
+    // a temporal point that, at its rate over the last three steps, would
+    // reach tol only after 3/4 of maxit + 1 solves (or not at all) leaves now
+    // when a second pass follows: that pass takes such points whatever this
+    // one gives them (a second eigenvalue nearly as close to the shift), so
+    // the steps would be wasted
+    bool slow = false;
+    if (pass2 && temporal && it >= 6 && rel >= 1e-12 && rel >= tol) {
+      const double rate = (back[2] > 0.0 && back[2] < 1e300) ? cbrt(rel / back[2]) : 1.0;
+      slow = !(rate < 1.0) || nsolve + log(tol / rel) / log(rate) > 0.75 * (maxit + 1);
+    }
+    back[2] = back[1]; back[1] = back[0]; back[0] = rel;
+    if (rel < tol || it >= maxit || stall >= 3 || slow) break;
+    // correction
+    shifted_solve<T, NB, NT>(P, A, ipiv, Lb, Ub, dr, dc, coef, ops, 0, L);
+    ++nsolve;
+    for (int r = t; r < n; r += NT) x[r] = x[r] - L.y[r];
+    __syncthreads();
+  }
+  // ---- getevec normalisation: first entry of largest modulus set to 1
+  double bm = -1.0;
+  int bi = 0x7fffffff;
+  for (int r = t; r < n; r += NT) { const double a = abs2(x[r]); if (a > bm) { bm = a; bi = r; } }
+  s_mx[t] = bm; s_ix[t] = bi;
+  __syncthreads();
+  if (t == 0) {
+    for (int k = 1; k < NT; ++k)
+      if (s_mx[k] > bm || (s_mx[k] == bm && s_ix[k] < bi)) { bm = s_mx[k]; bi = s_ix[k]; }
+    s_ix[0] = clamp_index(bi, n);     // all-NaN vector: no entry compares greater
+  }
+  __syncthreads();
+  const zd scale = x[s_ix[0]];
+  __syncthreads();
+  if (scale_ok(scale))
+    for (int r = t; r < n; r += NT) x[r] = cdiv(x[r], scale);
+  __syncthreads();
+  const double ff = farfield_ratio<NT>(P, x, L.red);
+  zd *out = evec + P.out_off;
+  for (int r = t; r < n; r += NT) {
+    out[r] = x[r];
+    if (!temporal) out[n + r] = lam * x[r];
+  }
+  if (t == 0) {
+    eval[pi] = lam;
+    resid[pi] = rel;
+    ffo[pi] = ff;
+    iters[pi] = nsolve;
+    flag[pi] = (rel < 1e-14) ? 0 : 2;
+  }
+  // ---- the second pass in the same work-group (pass2 == 2), for the points the
+  // rule of k_invit2 takes, with 2 maxit steps: it starts as soon as this
+  // point's first pass ends, beside the other points' first passes.  x1 is
+  // x (L.z), x2 L.res, a1 L.y, a2 L.rhs, b1 and b2 columns 2 and 3 of the
+  // point's vector block, tmp2 the 2 NT entries after the operator layout.
+  if (pass2 == 2 && lead2_needed(info[pi], P.itype, (rel < 1e-14) ? 0 : 2, want, P.alpha.x, lam, ff, nsolve, maxit)) {
+    zd *tmp2 = lds + op_lds_bytes(npadmax, nymax, 1, npadmax) / sizeof(zd);
+    const size_t nv = P.nvec;
+    invit2_point<T, NB, NT>(P, pi, mats, pivs, blks, scales, coef, ops, norms, 2 * maxit, tol, want, eval2, evec,
+                            resid2, ff2, iters, L, x, L.res, L.y, out + 2 * nv, L.rhs, out + 3 * nv, tmp2, s_mx,
+                            s_ix);
+  }
+}
+
+// The second pass as a kernel of its own, after k_invit: one work-group per
+// listed point p = sub[blockIdx.x], or per point order[blockIdx.x] that the
+// rule takes.  LDS: the full operator layout followed by six npadmax vectors
+// and 2 NT entries.
+template <typename T, int NB, int NT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_invit2(const PointDev *__restrict__ pts, const int *__restrict__ sub,
+                                               const cx<T> *__restrict__ mats, const int *__restrict__ pivs,
+                                               const cx<T> *__restrict__ blks, const double *__restrict__ scales,
+                                               const zd *__restrict__ coef, const double *__restrict__ ops,
+                                               const double *__restrict__ norms, int maxit, double tol, int want,
+                                               zd *__restrict__ eval2, zd *__restrict__ evec,
+                                               double *__restrict__ resid2, double *__restrict__ ff2,
+                                               int *__restrict__ iters, int npadmax, int nymax,
+                                               const int *__restrict__ order, const int *__restrict__ info,
+                                               const int *__restrict__ flag, const zd *__restrict__ eval1,
+                                               const double *__restrict__ ff1, int maxit1) {
+  extern __shared__ zd lds[];
+  // sub lists the points; without it work-group g takes point order[g] and
+  // runs only when the first pass (maxit1 steps) left that temporal point
+  // unconverged, not discrete (unless want == 1), or converged only after
+  // 3/4 of its steps -- the host applies the same rule to the same data
+  const int p = sub ? sub[blockIdx.x] : order[blockIdx.x];
+  if (!sub) {
+    const PointDev &Q = pts[p];
+    if (!lead2_needed(info[p], Q.itype, flag[p], want, Q.alpha.x, eval1[p], ff1[p], iters[p], maxit1)) return;
+  }
+  const PointDev &P = pts[p];
+  OpLds L(lds, P, npadmax, nymax, 1, npadmax);
+  zd *x1 = L.X2 + 5 * nymax;
+  zd *x2 = x1 + npadmax, *a1 = x2 + npadmax, *b1 = a1 + npadmax, *a2 = b1 + npadmax, *b2 = a2 + npadmax;
+  zd *tmp2 = b2 + npadmax;                     // 2 NT entries for the two-column solve
+  __shared__ double s_mx[NT];
+  __shared__ int s_ix[NT];
+  zd *out = evec + P.out_off;
+  for (int r = threadIdx.x; r < P.n; r += NT) x1[r] = out[r];
+  __syncthreads();
+  invit2_point<T, NB, NT>(P, p, mats, pivs, blks, scales, coef, ops, norms, maxit, tol, want, eval2, evec, resid2,
+                          ff2, iters, L, x1, x2, a1, b1, a2, b2, tmp2, s_mx, s_ix);
 }
 
 // Copy column sel[p] of each point's eigenvector block (P.out_off, nvec rows,

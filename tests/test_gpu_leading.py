@@ -227,3 +227,41 @@ def test_leading_invit_thesis_goldens(gpu_solver, ts_profile, case, mode):
     if case == "spatial":
         n = 5 * inp.ny
         assert np.linalg.norm(vec[n:] - res["evals"][0] * vec[:n]) <= 1e-10 * np.linalg.norm(vec[n:])
+
+
+def test_leading_fused_second_pass_matches_second_launch(gpu_solver):
+    """The second pass run inside the first-pass work-group as soon as a
+    point's first pass ends (the default) and run as a launch of its own after
+    the whole first pass (STAB_FUSE2=0) apply the same rule to the same
+    first-pass results and the same two-vector iteration: on 1024 points of
+    the benchmark sweep (some of which take the second pass) they agree on
+    every status and solve count, eigenvalues to 1e-12 and eigenvectors to
+    1e-10."""
+    import os
+    import bench
+    from stab_amd import _lib, learned
+    tab, al, be, re, ny = bench.make_workload(4096, 2024)
+    gpu_solver.set_profile(0, tab[:, 0], tab[:, 1:6])
+    idx = np.arange(1024)
+    pts = bench.make_points(al[idx], be[idx], re[idx], ny[idx], 8)
+    mlp = learned.ShiftMLP.load(gpu_solver)
+    om = learned.shifts(mlp, bench.features(al[idx], be[idx], re[idx]))
+    out = {}
+    try:
+        for fuse in ("1", "0"):
+            os.environ["STAB_FUSE2"] = fuse
+            out[fuse] = gpu_solver.solve_leading(pts, _lib.MODE_MIXED, _lib.LEAD_INVIT, maxit=20,
+                                                 fallback_shifts=om, shifts=om, nev=4, want_vectors=True)
+    finally:
+        os.environ.pop("STAB_FUSE2", None)
+    a, b = out["1"], out["0"]
+    second = int((a["iters"] > 21).sum())
+    rel = np.abs(a["evals"] - b["evals"]) / np.abs(b["evals"])
+    dv = max(float(np.max(np.abs(u - v))) for u, v in zip(a["evecs"], b["evecs"]))
+    bit = bool(np.array_equal(a["evals"], b["evals"]))
+    _say(f"fused second pass: {second} points took it; bitwise equal eigenvalues {bit}; "
+         f"max rel diff {rel.max():.1e}, eigenvector diff {dv:.1e}")
+    assert second >= 5
+    assert np.array_equal(a["info"], b["info"]) and np.array_equal(a["iters"], b["iters"])
+    assert rel.max() <= 1e-12 and dv <= 1e-10
+    assert (a["info"] == _lib.INFO_OK).sum() >= 1000
