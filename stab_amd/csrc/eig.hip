@@ -108,10 +108,14 @@ __device__ inline void solve_diag(const cx<T> *__restrict__ inv, int k0, int bs,
 }
 
 // y_r[r0..r1) -= A[r0..r1, k0..k0+bs) tmp[r NT + 0..bs) for r < NR (the matrix
-// is loaded once for all right-hand sides); r0, r1 multiples of E.
+// is loaded once for all right-hand sides); r0, r1 multiples of E.  PIPE
+// (the inverse-iteration kernels) issues each entry's load for the next pass
+// as soon as the entry is used, so loads stay in flight through the arithmetic
+// and the reductions; the other kernels, tighter on registers, load pass by
+// pass.  The arithmetic is the same either way.
 // Branch-free: column indexes past bs are clamped to bs - 1 (tmp is zero
 // there) and rows past r1 to the last row group (not written back).
-template <typename T, int NT, int NR>
+template <typename T, int NT, int NR, bool PIPE = false>
 __device__ inline void solve_update(const cx<T> *__restrict__ A, int lda, int k0, int bs, int r0, int r1,
                                     zd *const (&y)[NR], const zd *tmp) {
   typedef Row16<T> R;
@@ -123,12 +127,17 @@ __device__ inline void solve_update(const cx<T> *__restrict__ A, int lda, int k0
   int coff[16];                              // this lane's 16 columns, in V units
 #pragma unroll
   for (int j = 0; j < 16; ++j) coff[j] = (k0 + min(q * 16 + j, bs - 1)) * ldv;
-#pragma unroll 1
-  for (int rb = r0; rb < r1; rb += G * E) {
-    const int row = rb + g * E, rl = min(row, r1 - E);
-    V v[16];
+  V v[16];
+  auto load = [&](int rb) {
+    const int rl = min(rb + g * E, r1 - E);
 #pragma unroll
     for (int j = 0; j < 16; ++j) v[j] = __builtin_nontemporal_load(base + coff[j] + rl / E);
+  };
+  // one pass over rows rb .. rb + G E; with NEXT, each entry's register is
+  // reloaded with the next pass's entry (rows rn ..) as soon as it is used
+  auto pass = [&](int rb, int rn, auto next) {
+    constexpr bool NEXT = decltype(next)::value;
+    const int row = rb + g * E, rln = min(rn + g * E, r1 - E);
     zd a[NR][E];
 #pragma unroll
     for (int r = 0; r < NR; ++r)
@@ -142,6 +151,7 @@ __device__ inline void solve_update(const cx<T> *__restrict__ A, int lda, int k0
 #pragma unroll
         for (int e = 0; e < E; ++e) a[r][e] = cfma(a[r][e], R::get(v[j], e), mj);
       }
+      if (NEXT) v[j] = __builtin_nontemporal_load(base + coff[j] + rln / E);
     }
 #pragma unroll
     for (int r = 0; r < NR; ++r)
@@ -156,12 +166,28 @@ __device__ inline void solve_update(const cx<T> *__restrict__ A, int lda, int k0
 #pragma unroll
         for (int e = 0; e < E; ++e) y[r][row + e] = y[r][row + e] - a[r][e];
     }
+  };
+  if (!PIPE) {
+#pragma unroll 1
+    for (int rb = r0; rb < r1; rb += G * E) {
+      load(rb);
+      pass(rb, rb, std::false_type());
+    }
+    return;
   }
+  // the next pass's loads go out while this pass computes (in the registers
+  // this pass has just used, so no second buffer)
+  if (r0 >= r1) return;
+  load(r0);
+  int rb = r0;
+#pragma unroll 1
+  for (; rb + G * E < r1; rb += G * E) pass(rb, rb + G * E, std::true_type());
+  pass(rb, rb, std::false_type());
 }
 
 // NR right-hand sides y[r] (LDS, npad) solved together; tmp holds NR NT
 // entries, work npad.
-template <typename T, int NT, int NR>
+template <typename T, int NT, int NR, bool PIPE = false>
 __device__ __forceinline__ void lu_solve_n(const PointDev &P, const cx<T> *__restrict__ A, const int *__restrict__ ipiv,
                                            const cx<T> *__restrict__ blk, zd *const (&y)[NR], zd *tmp, zd *work) {
   static_assert(NT == 4 * SOLVE_BS, "four lanes per row of a 64-row block");
@@ -202,7 +228,7 @@ __device__ __forceinline__ void lu_solve_n(const PointDev &P, const cx<T> *__res
     }
     solve_diag<T, NT, NR>(Li + (size_t)b * BS * BS, k0, bs, y, tmp);
     if (k0 + bs < n) {
-      solve_update<T, NT, NR>(A, lda, k0, bs, k0 + bs, n, y, tmp);
+      solve_update<T, NT, NR, PIPE>(A, lda, k0, bs, k0 + bs, n, y, tmp);
       __syncthreads();
     }
   }
@@ -210,17 +236,17 @@ __device__ __forceinline__ void lu_solve_n(const PointDev &P, const cx<T> *__res
     const int k0 = b * BS, bs = min(BS, n - k0);
     solve_diag<T, NT, NR>(Ui + (size_t)b * BS * BS, k0, bs, y, tmp);
     if (k0 > 0) {
-      solve_update<T, NT, NR>(A, lda, k0, bs, 0, k0, y, tmp);
+      solve_update<T, NT, NR, PIPE>(A, lda, k0, bs, 0, k0, y, tmp);
       __syncthreads();
     }
   }
 }
 
-template <typename T, int NT>
+template <typename T, int NT, bool PIPE = false>
 __device__ __forceinline__ void lu_solve(const PointDev &P, const cx<T> *__restrict__ A, const int *__restrict__ ipiv,
                                          const cx<T> *__restrict__ blk, zd *y, zd *tmp, zd *work) {
   zd *const ys[1] = {y};
-  lu_solve_n<T, NT, 1>(P, A, ipiv, blk, ys, tmp, work);
+  lu_solve_n<T, NT, 1, PIPE>(P, A, ipiv, blk, ys, tmp, work);
 }
 
 // Block-wide reduction of NV complex partial sums; result broadcast to all.
@@ -292,14 +318,14 @@ __host__ __device__ inline size_t op_lds_bytes(int npadmax, int nymax, int full,
 // the correction is below 1e-14 of the solution or stops shrinking by half
 // (the FP64 rounding floor of the residual has been reached).  The contraction
 // per step is about kappa(M) u32.
-template <typename T, int NB, int NT>
+template <typename T, int NB, int NT, bool PIPE = false>
 __device__ bool shifted_solve(const PointDev &P, const cx<T> *A, const int *ipiv, const cx<T> *Lb, const cx<T> *Ub,
                               const double *dr, const double *dc, const zd *coef, const double *ops, int ir,
                               OpLds &L) {
   const int n = P.npad, t = threadIdx.x;
   for (int r = t; r < n; r += NT) L.y[r] = (r < P.n) ? L.rhs[r] * dr[r] : czero<double>();
   __syncthreads();
-  lu_solve<T, NT>(P, A, ipiv, Lb, L.y, L.tmp, L.w2);
+  lu_solve<T, NT, PIPE>(P, A, ipiv, Lb, L.y, L.tmp, L.w2);
   for (int r = t; r < n; r += NT) L.y[r] = L.y[r] * dc[r];
   __syncthreads();
   const bool adaptive = ir < 0;
@@ -310,7 +336,7 @@ __device__ bool shifted_solve(const PointDev &P, const cx<T> *A, const int *ipiv
     pencil_apply<NT>(P, coef, ops, FORM_SHIFTED, P.shift, L.y, L.res, L.X1, L.X2, L.red);
     for (int r = t; r < n; r += NT) L.res[r] = (r < P.n) ? (L.rhs[r] - L.res[r]) * dr[r] : czero<double>();
     __syncthreads();
-    lu_solve<T, NT>(P, A, ipiv, Lb, L.res, L.tmp, L.w2);
+    lu_solve<T, NT, PIPE>(P, A, ipiv, Lb, L.res, L.tmp, L.w2);
     zd nrm[2] = {czero<double>(), czero<double>()};
     for (int r = t; r < n; r += NT) {
       const zd d = L.res[r] * dc[r];
@@ -1055,7 +1081,7 @@ __device__ __forceinline__ void invit2_point(const PointDev &P, int p, const cx<
   const zd sg = P.shift;
   zd *out = evec + P.out_off;
   pencil_apply<NT>(P, coef, ops, FORM_BM, sg, x1, L.rhs, L.X1, L.X2, L.red);
-  shifted_solve<T, NB, NT>(P, A, ipiv, Lb, Ub, dr, dc, coef, ops, 0, L);
+  shifted_solve<T, NB, NT, true>(P, A, ipiv, Lb, Ub, dr, dc, coef, ops, 0, L);
   for (int r = t; r < n; r += NT) x2[r] = L.y[r];
   __syncthreads();
   int nsolve = 1;
@@ -1176,7 +1202,7 @@ __device__ __forceinline__ void invit2_point(const PointDev &P, int p, const cx<
     __syncthreads();
     {
       zd *const ys[2] = {a1, a2};
-      lu_solve_n<T, NT, 2>(P, A, ipiv, Lb, ys, tmp2, L.w2);
+      lu_solve_n<T, NT, 2, true>(P, A, ipiv, Lb, ys, tmp2, L.w2);
     }
     for (int r = t; r < n; r += NT) {
       x1[r] = x1[r] - a1[r] * dc[r];
@@ -1283,7 +1309,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
     for (int r = t; r < n; r += NT) L.rhs[r] = L.rhs[r] + (sg * 2.0) * L.res[r];
     __syncthreads();
   }
-  shifted_solve<T, NB, NT>(P, A, ipiv, Lb, Ub, dr, dc, coef, ops, 0, L);
+  shifted_solve<T, NB, NT, true>(P, A, ipiv, Lb, Ub, dr, dc, coef, ops, 0, L);
   for (int r = t; r < n; r += NT) x[r] = L.y[r];
   __syncthreads();
   int nsolve = 1;
@@ -1372,7 +1398,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
     back[2] = back[1]; back[1] = back[0]; back[0] = rel;
     if (rel < tol || it >= maxit || stall >= 3 || slow) break;
     // correction
-    shifted_solve<T, NB, NT>(P, A, ipiv, Lb, Ub, dr, dc, coef, ops, 0, L);
+    shifted_solve<T, NB, NT, true>(P, A, ipiv, Lb, Ub, dr, dc, coef, ops, 0, L);
     ++nsolve;
     for (int r = t; r < n; r += NT) x[r] = x[r] - L.y[r];
     __syncthreads();
