@@ -521,9 +521,12 @@ int launch_lu(stab_ctx *c, int npts, Batch &B, bool lapack_form = false, bool ti
     const LuSeg &g = segs[k];
     hipStream_t s = c->lus[k % ns];
     const int *ord = B.order + g.start;
+    const int top = B.h[B.horder[g.start]].npad;     // the segment's largest (largest-first order)
     for (const LuStep &st : g.steps) {
-      const int rc = (g.nt == 256) ? launch_lu_step<T, NB, 256>(c, s, g.R, st, ord, B, left)
-                                   : launch_lu_step<T, NB, 512>(c, s, g.R, st, ord, B, left);
+      // the panel's rows K0 .. top, R rows per thread
+      const int R = std::min(g.R, (top - st.K0 + g.nt - 1) / g.nt);
+      const int rc = (g.nt == 256) ? launch_lu_step<T, NB, 256>(c, s, R, st, ord, B, left)
+                                   : launch_lu_step<T, NB, 512>(c, s, R, st, ord, B, left);
       if (rc != STAB_OK) return rc;
     }
   }

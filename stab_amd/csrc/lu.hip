@@ -82,7 +82,9 @@ __device__ inline int net_perm(const int *ipiv, int k0, int w, int *srow, int *s
 //   k_lu_panel  one work-group per matrix whose order exceeds K0: factors the
 //               block column [K0, K0 + kb) (panels in registers, the in-block
 //               strips, on the matrix cores in FP32, the interchanges),
-//               stores the block's records, and forms X = inv(L11);
+//               stores the block's records, and forms X = inv(L11); thread t
+//               holds rows K0 + t + j NT, j < R, so R = ceil((n - K0) / NT)
+//               suffices and later block steps launch with a smaller R;
 //   k_lu_trail  one work-group per 64-column strip (32 in FP64) right of the
 //               block, for every such matrix: the block's row permutation,
 //               U12 = X A12 and A22 -= L21 U12 on the matrix cores.
@@ -142,7 +144,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(R <= 2 && NT
       // ---- load the panel rows owned by this thread
 #pragma unroll
       for (int j = 0; j < R; ++j) {
-        const int r = t + j * NT;
+        const int r = K0 + t + j * NT;
 #pragma unroll
         for (int c = 0; c < NB; ++c)
         {
@@ -169,7 +171,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(R <= 2 && NT
         int bi = 0x7fffffff;
 #pragma unroll
         for (int j = 0; j < R; ++j) {
-          const int r = t + j * NT;
+          const int r = K0 + t + j * NT;
           C pc = czero<T>();
 #pragma unroll
           for (int cc = 0; cc < NB; ++cc) if (cc == c) pc = Pn[j][cc];
@@ -186,14 +188,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(R <= 2 && NT
         // interchange.  The wave's candidate rows are its own rows, so the
         // owner of row bi is a lane of this wave.
         if (lane == 0) { s_rv[wv] = best; s_ri[wv] = bi; }
-        const int cj = col / NT, bj = (bi < n) ? bi / NT : -1;
+        const int cj = (col - K0) / NT, bj = (bi < n) ? (bi - K0) / NT : -1;
 #pragma unroll
         for (int j = 0; j < R; ++j) {
-          if (j == bj && (bi % NT) == t) {
+          if (j == bj && ((bi - K0) % NT) == t) {
 #pragma unroll
             for (int cc = 0; cc < NB; ++cc) s_cand2[c & 1][wv][cc] = Pn[j][cc];
           }
-          if (j == cj && (col % NT) == t) {
+          if (j == cj && ((col - K0) % NT) == t) {
 #pragma unroll
             for (int cc = 0; cc < NB; ++cc) s_crow[cc] = Pn[j][cc];
           }
@@ -209,15 +211,15 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(R <= 2 && NT
         const bool found = (bi >= col && bi < n);
         const int p = found ? bi : col;
         const C *s_prow = found ? s_cand2[c & 1][wb] : s_crow;
-        const int pj = p / NT;
+        const int pj = (p - K0) / NT;
         if (p != col) {
 #pragma unroll
           for (int j = 0; j < R; ++j) {
-            if (j == pj && (p % NT) == t) {
+            if (j == pj && ((p - K0) % NT) == t) {
 #pragma unroll
               for (int cc = 0; cc < NB; ++cc) Pn[j][cc] = s_crow[cc];
             }
-            if (j == cj && (col % NT) == t) {
+            if (j == cj && ((col - K0) % NT) == t) {
 #pragma unroll
               for (int cc = 0; cc < NB; ++cc) Pn[j][cc] = s_prow[cc];
             }
@@ -231,7 +233,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(R <= 2 && NT
           const C inv = cdiv(mk<T>(1, 0), piv);
 #pragma unroll
           for (int j = 0; j < R; ++j) {
-            const int r = t + j * NT;
+            const int r = K0 + t + j * NT;
             if (r > col && r < n) {
               C l = czero<T>();
 #pragma unroll
@@ -249,7 +251,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(R <= 2 && NT
       // ---- write the factored panel back; stage the diagonal block
 #pragma unroll
       for (int j = 0; j < R; ++j) {
-        const int r = t + j * NT;
+        const int r = K0 + t + j * NT;
         if (r >= k0 && r < n) {
 #pragma unroll
           for (int c = 0; c < NB; ++c) A[(size_t)(k0 + c) * lda + r] = Pn[j][c];
@@ -330,7 +332,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(R <= 2 && NT
           C *lst = s_u + (size_t)wv * 32 * NB;
 #pragma unroll
           for (int j = 0; j < R; ++j) {
-            const int w0 = wrow + j * NT;
+            const int w0 = K0 + wrow + j * NT;
             if (w0 >= n || w0 + 63 < k0 + NB) continue;           // wave-uniform
 #pragma unroll 1
             for (int half = 0; half < 2; ++half) {
@@ -385,9 +387,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(R <= 2 && NT
         for (int cg = 0; cg < sw; cg += G) {
 #pragma unroll
           for (int j = 0; j < R; ++j) {
-            const int w0 = wrow + j * NT;
+            const int w0 = K0 + wrow + j * NT;
             if (w0 >= n || w0 + 63 < k0 + NB) continue;
-            const int r = t + j * NT;
+            const int r = K0 + t + j * NT;
             const bool live = (r >= k0 + NB && r < n);
             C acc[G];
 #pragma unroll
