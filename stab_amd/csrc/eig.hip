@@ -1170,12 +1170,18 @@ __device__ __forceinline__ void invit2_point(const PointDev &P, int p, const cx<
       rel[k] = (den > 0.0) ? sqrt(nr[2 + k].x) / den : 1e300;
     }
     bool conv[2];
-    double ffk[2];
+    double ffk[2] = {1.0, 1.0};
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
       if (rel[k] < 0.5 * best[k]) { best[k] = rel[k]; stall[k] = 0; } else if (rel[k] < 1e-12) { ++stall[k]; }
       conv[k] = rel[k] < tol || stall[k] >= 3;
-      ffk[k] = farfield_ratio<NT>(P, k == 0 ? x1 : x2, L.red);
+    }
+    // the farfield ratios enter only the choice of the wanted pair and the
+    // early stop, which need a converged pair (conv comes from block-wide
+    // sums, so the branch is the same for the whole work-group)
+    if (conv[0] || conv[1]) {
+#pragma unroll
+      for (int k = 0; k < 2; ++k) ffk[k] = farfield_ratio<NT>(P, k == 0 ? x1 : x2, L.red);
     }
     int w;
     if (want == 1) {
