@@ -68,7 +68,7 @@ def test_leading_krylov_matches_qz(gpu_solver, sweep, krylov):
         x = so.normalize_evec(V[:, jl], 5 * int(ny[i]))
         assert np.max(np.abs(krylov["evecs"][i] - x)) <= 1e-6
         nfound += 1
-    assert nfound >= NPTS // 2
+    assert nfound == NPTS          # every point of this seeded sweep has a leading discrete mode
 
 
 @pytest.mark.parametrize("mode", [0, 1])

@@ -3,8 +3,8 @@ compressible Blasius profile, N = 128 (linearisation of order 1280), omega ~
 U[0.02, 0.15], beta = 0, Re ~ U[500, 2500] (seed 4), one launch per phase.
 
 * Plain FP64 shift-invert Krylov-Schur (stab_solve, the 4 pairs nearest
-  sigma = omega / 0.6 - 0.002 i): on three points every pair with an FP64
-  residual below 1e-10 matches the oracle's dense shift-invert eigenpairs
+  sigma = omega / 0.6 - 0.002 i): on nine points (every eighth and the
+  last) every pair with an FP64 residual below 1e-10 matches the oracle's dense shift-invert eigenpairs
   (eigenvalue 1e-8, getevec-normalised eigenvector 1e-6), at least two per
   point.  On many of these points the four nearest pairs reach into the
   discretised continuous spectrum near alpha = omega, dozens of eigenvalues
@@ -15,7 +15,7 @@ U[0.02, 0.15], beta = 0, Re ~ U[500, 2500] (seed 4), one launch per phase.
 * Residual inverse iteration with FP32 factors from the same shifts
   (stab_solve_leading, nearest-eigenvalue rule): every point it accepts
   returns the converged Krylov-Schur eigenvalue nearest the shift, to 1e-8
-  relative, with an FP64 residual below 1e-10.
+  relative, with an FP64 residual below 1e-10, and it accepts all 64.
 """
 import numpy as np
 import pytest
@@ -57,7 +57,7 @@ def test_spatial_sweep_krylov_matches_oracle(gpu_solver, spatial_sweep):
     import scipy.linalg as sl
     from oracle import stab_oracle as so
     tab, txts, sig, pts, ks = spatial_sweep
-    for i in (0, NPTS // 2, NPTS - 1):
+    for i in sorted(set(range(0, NPTS, 8)) | {NPTS - 1}):
         pb = so.setup(so.parse_input(txts[i]), tab[:, 0], tab[:, 1:6].copy(), spatial=True)
         Aa, Bb = so.spatial_linearization(*so.spatial_pencil(pb))
         th, V = sl.eig(sl.lu_solve(sl.lu_factor(Aa - sig[i] * Bb), Bb))
@@ -94,4 +94,4 @@ def test_spatial_sweep_invit_matches_krylov(gpu_solver, spatial_sweep):
         assert res["resid"][i] < 1e-10
         nok += 1
     _say(f"  {nok} of {NPTS} points: nearest eigenvalue equal to the Krylov-Schur one")
-    assert nok >= NPTS - 4
+    assert nok == NPTS
